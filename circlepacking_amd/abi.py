@@ -1,0 +1,118 @@
+"""ctypes mirror of include/cpl.h (structures, enums and configuration presets)."""
+import ctypes
+
+START, START_END_2D, START_END_3D, MIDDLE, END_2D, END_3D = range(6)
+FAMILY_NAMES = ["start", "start_end_2d", "start_end_3d", "middle", "end_2d", "end_3d"]
+IS_3D = {START: True, START_END_2D: False, START_END_3D: True, MIDDLE: True, END_2D: False, END_3D: True}
+
+V_PROVEN, V_CRITICAL, V_NOFIT, V_EMPTY, V_OOD, V_TRAP = range(6)
+VERDICT_NAMES = ["PROVEN", "CRITICAL", "NOFIT", "EMPTY", "OOD", "TRAP"]
+
+RF_CRITICAL, RF_NOFIT, RF_TRAP, RF_EMPTY, RF_OOD, RF_PROVEN, RF_NOFIT_FLAG = 1, 2, 4, 8, 16, 32, 64
+
+CPL_OK = 0
+ERR_NAMES = {-1: "HIP", -2: "ARG", -3: "TRAP", -4: "NOFIT", -5: "CAPACITY", -6: "NO_DEVICE"}
+MAX_ROUNDS = 16
+
+LAMBDA_MIN = 0.5
+LAMBDA_MAX = 0.9900000000000001
+
+# reference subdivision counts per family: la, r1, r2, r3
+REF_SUBDIV = {
+    START: (256, 256, 256, 64),
+    START_END_2D: (4096, 4096, 4096, 1),
+    START_END_3D: (256, 256, 256, 64),
+    MIDDLE: (512, 256, 256, 64),
+    END_2D: (4096, 1024, 1024, 1),
+    END_3D: (1024, 256, 128, 128),
+}
+
+PRIM_OPS = {
+    "add_rd": 0, "add_ru": 1, "sub_rd": 2, "sub_ru": 3, "mul_rd": 4, "mul_ru": 5,
+    "div_rd": 6, "div_ru": 7, "sqrt_rd": 8, "sqrt_ru": 9, "asin_rn": 10,
+    "dasin_rd": 11, "dasin_ru": 12, "lbrsp": 13, "div_rn": 14, "sqrt_rn": 15,
+}
+
+
+class Iv(ctypes.Structure):
+    _fields_ = [("lb", ctypes.c_double), ("ub", ctypes.c_double)]
+
+
+class RefineCfg(ctypes.Structure):
+    _fields_ = [
+        ("grid", ctypes.c_int32 * 4),
+        ("box_lb", ctypes.c_double * 4),
+        ("box_ub", ctypes.c_double * 4),
+        ("family_mask", ctypes.c_uint32),
+        ("clip", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("split_mode", ctypes.c_int32),
+        ("split_axes", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+    def families(self):
+        return [f for f in range(6) if self.family_mask & (1 << f)]
+
+    def n_roots(self):
+        n = 1
+        for g in self.grid:
+            n *= g
+        return n
+
+
+class RefineStats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("evaluations", ctypes.c_uint64 * MAX_ROUNDS),
+        ("verdicts", (ctypes.c_uint64 * 8) * MAX_ROUNDS),
+        ("digest", ctypes.c_uint64 * MAX_ROUNDS),
+        ("verdict_digest", ctypes.c_uint64 * MAX_ROUNDS),
+        ("total_evaluations", ctypes.c_uint64),
+        ("certified", ctypes.c_uint64),
+        ("unresolved", ctypes.c_uint64),
+        ("seconds", ctypes.c_double),
+        ("eval_seconds", ctypes.c_double),
+        ("peak_items", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        r = self.rounds
+        return {
+            "rounds": r,
+            "evaluations": list(self.evaluations[:r]),
+            "verdicts": [list(self.verdicts[d][:6]) for d in range(r)],
+            "digest": list(self.digest[:r]),
+            "verdict_digest": list(self.verdict_digest[:r]),
+            "total_evaluations": self.total_evaluations,
+            "certified": self.certified,
+            "unresolved": self.unresolved,
+        }
+
+
+def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_axes):
+    c = RefineCfg()
+    c.grid[:] = list(grid)
+    c.box_lb[:] = list(box_lb)
+    c.box_ub[:] = list(box_ub)
+    c.family_mask = sum(1 << f for f in families)
+    c.clip = int(clip)
+    c.max_depth = int(max_depth)
+    c.split_mode = int(split_mode)
+    c.split_axes = int(split_axes)
+    return c
+
+
+def cfg_small_2param(raw=False, n=256, max_depth=2):
+    """2-parameter radius domain [0.05,0.5]^2 at lambda = 0.5, n x n grid, 2x2 splits."""
+    return make_cfg((1, n, n, 1), (LAMBDA_MIN, 0.05, 0.05, 0.0), (LAMBDA_MIN, 0.5, 0.5, 1.0),
+                    (START_END_2D, END_2D), not raw, max_depth, 1, 0b0110)
+
+
+def cfg_4param(n=64, max_depth=6, families=(START, START_END_3D, MIDDLE, END_3D), grid=None):
+    """4-parameter box: lambda in [lambda_min, lambda_max], r1, r2 in [0.05, 0.5],
+    r3 position in [0, 1]; widest-axis bisection of undecided cells."""
+    g = grid or (n, n, n, n)
+    return make_cfg(g, (LAMBDA_MIN, 0.05, 0.05, 0.0), (LAMBDA_MAX, 0.5, 0.5, 1.0),
+                    families, True, max_depth, 0, 0b1111)

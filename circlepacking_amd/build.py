@@ -1,0 +1,62 @@
+"""Build step of the lane-packing prover: compiles libcpl_hip.so for gfx950.
+
+    python -m circlepacking_amd.build          # HIP library only
+The library is built in-tree (circlepacking_amd/libcpl_hip.so) so that it
+travels with the source tree to the machine that runs it.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcpl_hip.so")
+SOURCES = ["cpl_hip.hip"]
+HEADERS = ["cpl_round.hpp", "cpl_iv.hpp", "cpl_asin.hpp", "cpl_lane.hpp", "cpl_kernels.hpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # no implicit contraction: every fused multiply-add is written explicitly
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    # packed FP32 instructions are not used on this hardware pool
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-function",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    return "hipcc"
+
+
+def build_hip(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + srcs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    err = "\n".join(l for l in res.stderr.splitlines() if noise not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
+    if err.strip() and verbose:
+        print(err, file=sys.stderr)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def build_driver(verbose=False):
+    """C++ prover driver (added with the driver sources)."""
+    return None
+
+
+if __name__ == "__main__":
+    print(build_hip(verbose=True))

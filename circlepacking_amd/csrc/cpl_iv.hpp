@@ -1,0 +1,120 @@
+// Device interval type for the lane-packing prover.
+//
+// Mirrors algcuda::Interval<double> / Uncertain<bool>
+// (algcuda/cuda-utils/cpp/include/algcuda/interval.hpp:32-242) and the device
+// arithmetic of algcuda/cuda-utils/cuh/include/algcuda/interval.cuh:30-355,
+// on top of the MODE-register primitives of cpl_round.hpp.  Same POD layout
+// {double lb, ub}; same operator set and semantics, so the criteria read like
+// the reference's.  trap() of the reference becomes a per-thread trap code.
+#pragma once
+
+#include "cpl_round.hpp"
+
+namespace cpl {
+
+struct UB {
+  bool lb, ub;  // lb: definitely, ub: possibly (interval.hpp:32-82)
+  __device__ UB operator!() const { return UB{!ub, !lb}; }
+};
+__device__ __forceinline__ bool definitely(UB b) { return b.lb; }
+__device__ __forceinline__ bool possibly(UB b) { return b.ub; }
+
+struct IV {
+  double lb, ub;
+  IV() = default;
+  __host__ __device__ constexpr IV(double l, double u) : lb(l), ub(u) {}
+
+  __device__ bool empty() const { return lb > ub; }
+  __device__ void tighten_lb(double v) { if (lb < v) lb = v; }
+  __device__ void tighten_ub(double v) { if (v < ub) ub = v; }
+  __device__ IV& do_join(const IV& o) {
+    if (o.lb < lb) lb = o.lb;
+    if (o.ub > ub) ub = o.ub;
+    return *this;
+  }
+  __device__ IV intersect(const IV& o) const { return IV(lb < o.lb ? o.lb : lb, ub > o.ub ? o.ub : ub); }
+  __device__ IV abs() const {  // interval.hpp:216-232
+    double l = 0.0, u = 0.0;
+    if (ub > 0) u = ub; else l = -ub;
+    if (lb < 0) u = u < -lb ? -lb : u; else l = lb;
+    return IV(l, u);
+  }
+  __device__ IV square() const {  // interval.cuh:352-355
+    IV a = abs();
+    IV r;
+    mul_pair(a.lb, a.lb, a.ub, a.ub, r.lb, r.ub);
+    return r;
+  }
+};
+
+__device__ __forceinline__ IV operator+(const IV& a, const IV& b) {
+  IV r;
+  add_pair(a.lb, b.lb, a.ub, b.ub, r.lb, r.ub);
+  return r;
+}
+__device__ __forceinline__ IV operator+(const IV& a, double b) { return a + IV(b, b); }
+__device__ __forceinline__ IV operator+(double a, const IV& b) { return b + IV(a, a); }
+__device__ __forceinline__ IV operator-(const IV& a, const IV& b) {
+  IV r;
+  sub_pair(a.lb, b.ub, a.ub, b.lb, r.lb, r.ub);
+  return r;
+}
+__device__ __forceinline__ IV operator-(const IV& a, double b) { return a - IV(b, b); }
+__device__ __forceinline__ IV operator-(double a, const IV& b) { return IV(a, a) - b; }
+
+// Interval x scalar (interval.cuh:185-197); for n >= 0 the upper bound is
+// rounded up (the reference rounds it down; identical for the power-of-two
+// factors 0.25, 0.5, 2 of every call site on this path).
+__device__ __forceinline__ IV operator*(const IV& a, double n) {
+  IV r;
+  if (n >= 0) mul_pair(a.lb, n, a.ub, n, r.lb, r.ub);
+  else mul_pair(a.ub, n, a.lb, n, r.lb, r.ub);
+  return r;
+}
+__device__ __forceinline__ IV operator*(double n, const IV& a) { return a * n; }
+
+// Interval x interval, 8-product form (interval.cuh:171-182).
+__device__ __forceinline__ IV operator*(const IV& a, const IV& b) {
+  double l1, l2, l3, l4, u1, u2, u3, u4;
+  mul_pair(a.lb, b.lb, a.lb, b.lb, l1, u1);
+  mul_pair(a.lb, b.ub, a.lb, b.ub, l2, u2);
+  mul_pair(a.ub, b.lb, a.ub, b.lb, l3, u3);
+  mul_pair(a.ub, b.ub, a.ub, b.ub, l4, u4);
+  double ll1 = l1 < l2 ? l1 : l2, ll2 = l3 < l4 ? l3 : l4;
+  double uu1 = u1 > u2 ? u1 : u2, uu2 = u3 > u4 ? u3 : u4;
+  return IV(ll1 < ll2 ? ll1 : ll2, uu1 > uu2 ? uu1 : uu2);
+}
+
+// Interval / interval with sign cases (interval.cuh:277-316).
+__device__ __forceinline__ IV operator/(const IV& a, const IV& o) {
+  double l, u;
+  if (o.lb > 0) {
+    if (a.lb >= 0) { l = div_rd(a.lb, o.ub); u = div_ru(a.ub, o.lb); }
+    else if (a.ub <= 0) { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.ub); }
+    else { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.lb); }
+  } else if (o.ub < 0) {
+    if (a.lb >= 0) { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.lb); }
+    else if (a.ub <= 0) { l = div_rd(a.ub, o.lb); u = div_ru(a.lb, o.ub); }
+    else { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.ub); }
+  } else {
+    u = __builtin_huge_val();
+    l = -u;
+  }
+  return IV(l, u);
+}
+
+__device__ __forceinline__ UB operator<(const IV& a, const IV& b) { return UB{a.ub < b.lb, a.lb < b.ub}; }
+__device__ __forceinline__ UB operator<=(const IV& a, const IV& b) { return UB{a.ub <= b.lb, a.lb <= b.ub}; }
+__device__ __forceinline__ UB operator>(const IV& a, const IV& b) { return b < a; }
+__device__ __forceinline__ UB operator>=(const IV& a, const IV& b) { return b <= a; }
+
+// get_subinterval (subintervals.hpp:31-42) in round-to-nearest with the
+// multiply-add fused, as nvcc compiles lb + diff*i.
+__device__ __forceinline__ IV get_subinterval(IV iv, int i, int n) {
+  double diff = (iv.ub - iv.lb) / (double)n;
+  double l = __builtin_fma(diff, (double)i, iv.lb);
+  double u = (i + 1 == n) ? iv.ub : __builtin_fma(diff, (double)(i + 1), iv.lb);
+  return IV(l, u);
+}
+
+}  // namespace cpl

@@ -1,0 +1,213 @@
+// Directed-rounding FP64 primitives for gfx950 (CDNA4).
+//
+// CUDA has per-instruction rounding (__dadd_rd, __dmul_ru, ...).  CDNA has a
+// per-wave rounding MODE instead: MODE[3:2] is the FP64/FP16 rounding field
+// (0 nearest-even, 1 toward +inf, 2 toward -inf, 3 toward zero), MODE[1:0] the
+// FP32 one.  Every directed operation here is one asm statement that
+//     s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), <RD|RU>
+//     <the IEEE instruction(s)>
+//     s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0
+// so the wave is back in round-to-nearest whenever compiler-generated code
+// runs.  Because each statement is self-contained and a pure function of its
+// operands, the compiler may schedule, hoist or merge them freely.
+//
+// Hazards: two s_setreg of the same hardware register need >= 2 wait states
+// between them (the rule the compiler's hazard recognizer applies to its own
+// s_setreg sequences).  Every statement therefore starts with s_nop 1 (a
+// preceding statement may have ended with an s_setreg) and pads a single
+// instruction between its own s_setreg writes with s_nop 0.
+//
+// add, sub and mul are single IEEE instructions, correctly rounded in the
+// selected mode.  Division and square root are multi-instruction sequences
+// whose directed-mode results are NOT correctly rounded, so they are computed
+// as: q = RN(a/b) by the compiler's IEEE sequence, then one fused
+// multiply-add pair executed in the target mode,
+//     r = fma(-q, b, a)          (the exact remainder, rounded toward the target)
+//     result = fma(r, y, q)      (y a power of two ~ 1/(2|b|))
+// whose directed rounding lands on q or its neighbour exactly as the rounded
+// true quotient does (Markstein-style correction; see div_dir below).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cpl {
+
+#define CPL_MODE_RD "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 2\n\t"
+#define CPL_MODE_RU "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 1\n\t"
+#define CPL_MODE_RN "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0\n\t"
+
+// ---- one instruction in one mode -------------------------------------------
+#define CPL_DIR_BINOP(NAME, MODE, INSN)                                                      \
+  __device__ __forceinline__ double NAME(double a, double b) {                              \
+    double r;                                                                               \
+    asm("s_nop 1\n\t" MODE INSN " %0, %1, %2\n\ts_nop 0\n\t" CPL_MODE_RN                    \
+        : "=v"(r)                                                                           \
+        : "v"(a), "v"(b));                                                                  \
+    return r;                                                                               \
+  }
+CPL_DIR_BINOP(add_rd, CPL_MODE_RD, "v_add_f64")
+CPL_DIR_BINOP(add_ru, CPL_MODE_RU, "v_add_f64")
+CPL_DIR_BINOP(mul_rd, CPL_MODE_RD, "v_mul_f64")
+CPL_DIR_BINOP(mul_ru, CPL_MODE_RU, "v_mul_f64")
+#undef CPL_DIR_BINOP
+
+__device__ __forceinline__ double sub_rd(double a, double b) {
+  double r;
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_add_f64 %0, %1, -%2\n\ts_nop 0\n\t" CPL_MODE_RN : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double sub_ru(double a, double b) {
+  double r;
+  asm("s_nop 1\n\t" CPL_MODE_RU "v_add_f64 %0, %1, -%2\n\ts_nop 0\n\t" CPL_MODE_RN : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// ---- interval-shaped pairs: lb in RD, ub in RU, one statement ---------------
+// [rd(al + bl), ru(au + bu)]
+__device__ __forceinline__ void add_pair(double al, double bl, double au, double bu, double& l, double& u) {
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_add_f64 %0, %2, %3\n\ts_nop 0\n\t" CPL_MODE_RU
+      "v_add_f64 %1, %4, %5\n\ts_nop 0\n\t" CPL_MODE_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bl), "v"(au), "v"(bu));
+}
+// [rd(al - bu), ru(au - bl)]
+__device__ __forceinline__ void sub_pair(double al, double bu, double au, double bl, double& l, double& u) {
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_add_f64 %0, %2, -%3\n\ts_nop 0\n\t" CPL_MODE_RU
+      "v_add_f64 %1, %4, -%5\n\ts_nop 0\n\t" CPL_MODE_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bu), "v"(au), "v"(bl));
+}
+// [rd(al * bl), ru(au * bu)]
+__device__ __forceinline__ void mul_pair(double al, double bl, double au, double bu, double& l, double& u) {
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_mul_f64 %0, %2, %3\n\ts_nop 0\n\t" CPL_MODE_RU
+      "v_mul_f64 %1, %4, %5\n\ts_nop 0\n\t" CPL_MODE_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bl), "v"(au), "v"(bu));
+}
+
+// ---- bit helpers ------------------------------------------------------------
+__device__ __forceinline__ uint64_t dbits(double d) { return __double_as_longlong(d); }
+__device__ __forceinline__ double dfrom(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ int dexp_field(double d) { return (int)((dbits(d) >> 52) & 0x7FF); }
+// 2^k for -1022 <= k <= 1023
+__device__ __forceinline__ double pow2(int k) { return dfrom((uint64_t)(k + 1023) << 52); }
+
+// ---- division ---------------------------------------------------------------
+// Correction step of a directed division, both fmas in the target mode:
+//   r = fma(-q, b, a), res = fma(r, y, q).
+// With b > 0, q = RN(a/b) and y = 2^-(e_b+1) (so y*b in [1/2, 1)), the value
+// q + r*y lies strictly between q and the true quotient whenever they differ,
+// hence rounds exactly as the true quotient does; r rounded toward the target
+// keeps the sign needed when the remainder is not representable.
+#define CPL_DIV_FIX(NAME, MODE)                                                              \
+  __device__ __forceinline__ double NAME(double q, double b, double a, double y) {          \
+    double r, res;                                                                          \
+    asm("s_nop 1\n\t" MODE "v_fma_f64 %1, -%2, %3, %4\n\t"                                   \
+        "v_fma_f64 %0, %1, %5, %2\n\t" CPL_MODE_RN                                           \
+        : "=&v"(res), "=&v"(r)                                                              \
+        : "v"(q), "v"(b), "v"(a), "v"(y));                                                  \
+    return res;                                                                             \
+  }
+CPL_DIV_FIX(div_fix_rd, CPL_MODE_RD)
+CPL_DIV_FIX(div_fix_ru, CPL_MODE_RU)
+#undef CPL_DIV_FIX
+
+// Scalar multiply of a possibly-tiny double by a power of two in the target
+// mode (used to bring a scaled quotient back into the subnormal range).
+#define CPL_SCALE_DIR(NAME, MODE)                                                            \
+  __device__ __forceinline__ double NAME(double x, double p2) {                              \
+    double r;                                                                               \
+    asm("s_nop 1\n\t" MODE "v_mul_f64 %0, %1, %2\n\ts_nop 0\n\t" CPL_MODE_RN                 \
+        : "=v"(r)                                                                           \
+        : "v"(x), "v"(p2));                                                                 \
+    return r;                                                                               \
+  }
+CPL_SCALE_DIR(scale_rd, CPL_MODE_RD)
+CPL_SCALE_DIR(scale_ru, CPL_MODE_RU)
+#undef CPL_SCALE_DIR
+
+// Directed quotient with b > 0 normalised, both finite and nonzero, and every
+// exponent inside the "comfortable" range so the remainder is exact:
+// |a|, |b|, |q| in [2^-960, 2^960].
+template <bool UP>
+__device__ __forceinline__ double div_core(double a, double b, double q) {
+  double y = pow2(1022 - dexp_field(b));  // 2^-(e_b+1): e_b = field - 1023
+  return UP ? div_fix_ru(q, b, a, y) : div_fix_rd(q, b, a, y);
+}
+
+__device__ __forceinline__ bool exp_comfortable(double d) {
+  unsigned e = (unsigned)dexp_field(d);
+  return e - 63u < 1921u;  // field in [63, 1983]  <=>  |d| in [2^-960, 2^961)
+}
+
+// Slow path: zeros, infinities, NaN, overflow, subnormal operands or results.
+// Both operands are scaled by exact powers of two into [1, 2), the scaled
+// quotient is rounded in the target mode (exact remainder there), and the
+// power of two is re-applied in the target mode.  Rounding twice in the same
+// direction onto nested grids equals rounding once, so underflowing and
+// overflowing quotients come out exactly as IEEE directed division gives them.
+template <bool UP>
+__device__ __attribute__((noinline)) double div_slow(double a, double b) {
+  double q = a / b;
+  if (__builtin_isnan(q) || a == 0.0 || __builtin_isinf(a) || b == 0.0 || __builtin_isinf(b)) return q;
+  if (b < 0.0) { a = -a; b = -b; }
+  int shift = 0;  // true quotient = (a / b) * 2^shift after the scalings below
+  if (dexp_field(b) == 0) { b *= 0x1p+600; shift += 600; }
+  if (dexp_field(a) == 0) { a *= 0x1p+600; shift -= 600; }
+  int eb = dexp_field(b) - 1023, ea = dexp_field(a) - 1023;
+  b *= pow2(-eb);
+  a *= pow2(-ea);
+  shift += ea - eb;
+  double qs = a / b;  // |qs| in (1/2, 2)
+  double ds = UP ? div_fix_ru(qs, b, a, 0.5) : div_fix_rd(qs, b, a, 0.5);
+  while (shift > 1000) { ds = UP ? scale_ru(ds, 0x1p+1000) : scale_rd(ds, 0x1p+1000); shift -= 1000; }
+  while (shift < -1000) { ds = UP ? scale_ru(ds, 0x1p-1000) : scale_rd(ds, 0x1p-1000); shift += 1000; }
+  return UP ? scale_ru(ds, pow2(shift)) : scale_rd(ds, pow2(shift));
+}
+
+template <bool UP>
+__device__ __forceinline__ double div_dir(double a, double b) {
+  double q = a / b;  // IEEE round-to-nearest
+  if (__builtin_expect(exp_comfortable(a) & exp_comfortable(b) & exp_comfortable(q), 1)) {
+    double sa = b < 0.0 ? -a : a;
+    double sb = b < 0.0 ? -b : b;
+    double sq = b < 0.0 ? -q : q;
+    return div_core<UP>(sa, sb, sq);
+  }
+  return div_slow<UP>(a, b);
+}
+__device__ __forceinline__ double div_rd(double a, double b) { return div_dir<false>(a, b); }
+__device__ __forceinline__ double div_ru(double a, double b) { return div_dir<true>(a, b); }
+
+// ---- square root ------------------------------------------------------------
+// s = RN(sqrt x); r = fma(-s, s, x) and fma(r, y, s) in the target mode with
+// y = 2^-(e_s+2), so r*y lies between 0 and sqrt(x) - s.
+#define CPL_SQRT_FIX(NAME, MODE)                                                             \
+  __device__ __forceinline__ double NAME(double s, double x, double y) {                    \
+    double r, res;                                                                          \
+    asm("s_nop 1\n\t" MODE "v_fma_f64 %1, -%2, %2, %3\n\t"                                   \
+        "v_fma_f64 %0, %1, %4, %2\n\t" CPL_MODE_RN                                           \
+        : "=&v"(res), "=&v"(r)                                                              \
+        : "v"(s), "v"(x), "v"(y));                                                          \
+    return res;                                                                             \
+  }
+CPL_SQRT_FIX(sqrt_fix_rd, CPL_MODE_RD)
+CPL_SQRT_FIX(sqrt_fix_ru, CPL_MODE_RU)
+#undef CPL_SQRT_FIX
+
+template <bool UP>
+__device__ __forceinline__ double sqrt_dir(double x) {
+  if (!(x > 0.0) || __builtin_isinf(x)) return __builtin_sqrt(x);  // 0, -0, inf, NaN, negative
+  int k = 0;
+  if (dexp_field(x) < 64) { x *= 0x1p+200; k = -100; }  // tiny or subnormal: sqrt(x 2^200) = sqrt(x) 2^100
+  double s = __builtin_sqrt(x);
+  double y = pow2(1023 - 2 - dexp_field(s));
+  double r = UP ? sqrt_fix_ru(s, x, y) : sqrt_fix_rd(s, x, y);
+  if (k) r = UP ? scale_ru(r, pow2(k)) : scale_rd(r, pow2(k));
+  return r;
+}
+__device__ __forceinline__ double sqrt_rd(double x) { return sqrt_dir<false>(x); }
+__device__ __forceinline__ double sqrt_ru(double x) { return sqrt_dir<true>(x); }
+
+}  // namespace cpl

@@ -1,0 +1,173 @@
+/* cpl.h — C ABI of the MI355X lane-packing prover (libcpl_hip.so).
+ *
+ * The reference (phillip-keldenich/circlepacking, code/) exposes this path as
+ * C++ symbols inside one CUDA executable: kernels of the form
+ *     __global__ void kernel(IV outer_la, Atomic_ordered_double* first, bool output_criticals)
+ * launched by run_kernel_until_no_criticals (lane_packing/run_kernel.cuh:61-132)
+ * for six families (start_configuration.cu, middle_configuration.cu,
+ * end_configuration.cu).  This header is the drop-in boundary that replaces
+ * those launches: plain C types, caller-owned host buffers, 0 on success and a
+ * negative CPL_ERR_* code on failure (no exceptions cross it; the C++ driver
+ * re-throws std::system_error as the reference's throw_if_cuda_error does,
+ * algcuda/cuda-cpp-wrapper/include/algcuda/error.hpp:84-98).
+ *
+ * Threading: one library context per process and device; calls are
+ * serialised on one HIP stream owned by the library; not reentrant (the
+ * reference driver keeps global SIGINT state and is not reentrant either,
+ * run_kernel.cuh:38-39).
+ */
+#ifndef CPL_H_INCLUDED
+#define CPL_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Interval of doubles; layout-identical to algcuda::Interval<double>
+ * (algcuda/cuda-utils/cpp/include/algcuda/interval.hpp:84-242, two doubles lb, ub). */
+typedef struct {
+  double lb, ub;
+} cpl_iv;
+
+/* The six lane-packing kernels of the reference, one per configuration family. */
+typedef enum {
+  CPL_START = 0,        /* start_configuration.cu:114-167  kernel_find_critical_intervals_start_configuration */
+  CPL_START_END_2D = 1, /* start_configuration.cu:190-215  ..._start_end_configuration_2d */
+  CPL_START_END_3D = 2, /* start_configuration.cu:267-298  ..._start_end_configuration_3d */
+  CPL_MIDDLE = 3,       /* middle_configuration.cu:156-205 ..._middle_configuration */
+  CPL_END_2D = 4,       /* end_configuration.cu:103-146    ..._end_configuration_2d */
+  CPL_END_3D = 5,       /* end_configuration.cu:212-268    ..._end_configuration_3d */
+  CPL_NUM_FAMILIES = 6
+} cpl_family;
+
+/* Per-cell verdict byte: bits 0-2 = verdict, bit 3 = "possibly does not fit"
+ * flag of the start family (the reference prints and keeps evaluating,
+ * start_configuration.cu:131-135), bits 4-7 = detail (trap code for
+ * CPL_V_TRAP, 1-based failing criterion for CPL_V_CRITICAL). */
+enum {
+  CPL_V_PROVEN = 0,   /* every criterion of the family holds on the cell */
+  CPL_V_CRITICAL = 1, /* some criterion fails: the reference's store_min(la.lb) */
+  CPL_V_NOFIT = 2,    /* middle/end fit check fires: the reference traps */
+  CPL_V_EMPTY = 3,    /* tightening empties r1 or r2 (end_configuration.cu:232-234) */
+  CPL_V_OOD = 4,      /* a radius lies outside the family's admissible range */
+  CPL_V_TRAP = 5      /* a device consistency check of the reference would trap() */
+};
+/* Trap codes (detail nibble of CPL_V_TRAP). */
+enum {
+  CPL_TRAP_ASIN_DOMAIN = 1,       /* interval.cuh:617-621 */
+  CPL_TRAP_OBTUSE_EMPTY_X = 2,    /* collision_angles.cu:58-62 */
+  CPL_TRAP_OBTUSE_EMPTY_QUOT = 3, /* collision_angles.cu:71-75 */
+  CPL_TRAP_ACUTE_EMPTY_X = 4,     /* collision_angles.cu:91-101 */
+  CPL_TRAP_SQRT_DOMAIN = 5        /* interval.cuh:339-343 */
+};
+
+/* Error codes. */
+enum {
+  CPL_OK = 0,
+  CPL_ERR_HIP = -1,      /* a HIP runtime call failed; see cpl_last_error() */
+  CPL_ERR_ARG = -2,      /* invalid argument */
+  CPL_ERR_TRAP = -3,     /* a cell hit a reference trap condition (replaces trap()) */
+  CPL_ERR_NOFIT = -4,    /* middle/end fit check fired (the reference traps) */
+  CPL_ERR_CAPACITY = -5, /* a worklist outgrew its device allocation */
+  CPL_ERR_NO_DEVICE = -6
+};
+
+/* Grid axes of a synthetic refinement run. */
+enum { CPL_AXIS_LA = 0, CPL_AXIS_R1 = 1, CPL_AXIS_R2 = 2, CPL_AXIS_P3 = 3, CPL_NUM_AXES = 4 };
+
+#define CPL_MAX_ROUNDS 16
+
+/* Synthetic refinement run (a worklist of interval cells, one kernel launch
+ * per family and round).  Axis la holds the lambda interval, r1 and r2 the
+ * radius intervals, p3 a position in [0,1] that selects r3 inside the
+ * family's admissible r3 range (for a root cell it reproduces the reference's
+ * nested get_subinterval rule, subintervals.hpp:31-42). */
+typedef struct {
+  int32_t grid[CPL_NUM_AXES]; /* root cells per axis */
+  double box_lb[CPL_NUM_AXES], box_ub[CPL_NUM_AXES];
+  uint32_t family_mask;       /* bit f: evaluate family f */
+  int32_t clip;               /* 1: intersect radii with the admissible ranges (else raw box) */
+  int32_t max_depth;          /* number of refinement rounds after the root round (< CPL_MAX_ROUNDS) */
+  int32_t split_mode;         /* 0: bisect the widest eligible axis; 1: bisect all eligible axes */
+  uint32_t split_axes;        /* bitmask of eligible axes */
+  int32_t reserved[7];
+} cpl_refine_cfg;
+
+/* Per (root cell, family) outcome flags. */
+enum {
+  CPL_RF_CRITICAL = 1, /* a leaf stayed critical at max depth */
+  CPL_RF_NOFIT = 2,
+  CPL_RF_TRAP = 4,
+  CPL_RF_EMPTY = 8,
+  CPL_RF_OOD = 16,
+  CPL_RF_PROVEN = 32,
+  CPL_RF_NOFIT_FLAG = 64
+};
+
+typedef struct {
+  int32_t rounds;                                  /* rounds executed */
+  int32_t reserved0;
+  uint64_t evaluations[CPL_MAX_ROUNDS];            /* (cell, family) items evaluated per round */
+  uint64_t verdicts[CPL_MAX_ROUNDS][8];            /* per round, count of each verdict code */
+  uint64_t digest[CPL_MAX_ROUNDS];                 /* sum of item hashes per round (order independent) */
+  uint64_t verdict_digest[CPL_MAX_ROUNDS];         /* sum of hash(item, verdict byte) per round */
+  uint64_t total_evaluations;
+  uint64_t certified;                              /* leaves settled PROVEN, EMPTY or OOD */
+  uint64_t unresolved;                             /* leaves CRITICAL at max depth, NOFIT or TRAP */
+  double seconds;                                  /* device time of all rounds (HIP events) */
+  double eval_seconds;                             /* device time of the criteria kernels only */
+  uint64_t peak_items;                             /* largest worklist held on the device */
+} cpl_refine_stats;
+
+/* ---- lifecycle --------------------------------------------------------- */
+/* Create the library context on the current HIP device (no counterpart in the
+ * reference, which allocates per driver call: run_kernel.cuh:64). */
+int cpl_init(void);
+int cpl_shutdown(void);
+/* Message of the last failing call (thread-local, never NULL). */
+const char* cpl_last_error(void);
+
+/* ---- the reference kernel contract --------------------------------------- */
+/* One launch of family f over outer_la on the reference grid (subdivision
+ * counts of start_configuration.cu:41-56, middle_configuration.cu:39-47,
+ * end_configuration.cu:39-53), as run_kernel.cuh:81-84 does it: writes the
+ * smallest la.lb of a cell failing a criterion, or DBL_MAX if none fails.
+ * subdiv: NULL for the reference counts, else 4 counts la, r1, r2, r3 (r3
+ * ignored by 2-d families) for truncated runs.  With output_criticals, every
+ * critical cell is appended to the text buffer in the reference's %.19g
+ * format (the reference printf lines); out_text may be NULL.
+ * Returns CPL_ERR_TRAP / CPL_ERR_NOFIT where the reference would trap. */
+int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, int output_criticals,
+                          double* first_problematic_out, uint64_t* cells_evaluated_out,
+                          char* out_text, size_t out_text_cap);
+
+/* ---- per-cell evaluation (criterion interface) -------------------------- */
+/* Verdicts of family f for n cells given as 8 host arrays of n doubles
+ * (la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, p3.lb, p3.ub).  Equivalent to
+ * evaluating the reference's handle_* criteria with the kernel-body gating. */
+int cpl_eval_cells(int family, int clip, const double* const soa[8], size_t n, uint8_t* verdicts);
+
+/* Reference kernel body on explicit cells: la, r1_, r2_ (grid subintervals
+ * before tightening) and r3, 8 doubles per cell in that order (AoS). */
+int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* verdicts);
+
+/* ---- synthetic refinement ---------------------------------------------- */
+/* Runs the root round plus up to cfg->max_depth refinement rounds on the
+ * device.  root_flags (may be NULL): n_roots * popcount(family_mask) bytes,
+ * CPL_RF_* per (root, family) in increasing family order. */
+int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags);
+
+/* ---- diagnostics (no reference counterpart; used by the parity tests) --- */
+/* Elementwise primitive op on n host doubles: 0 add_rd 1 add_ru 2 sub_rd 3 sub_ru
+ * 4 mul_rd 5 mul_ru 6 div_rd 7 div_ru 8 sqrt_rd 9 sqrt_ru 10 asin_rn
+ * 11 dasin_rd 12 dasin_ru 13 lower_bound_radius_single_placement(a)
+ * 14 div_rn 15 sqrt_rn. */
+int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

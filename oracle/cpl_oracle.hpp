@@ -1,0 +1,800 @@
+// CPU oracle for the lane-packing prover — TEST INFRASTRUCTURE ONLY.
+//
+// This header restates, formula for formula, the reference prover's interval
+// arithmetic, lane geometry and criteria so that tests and the CPU-baseline
+// timing can check the HIP product against it.  Nothing in the product links,
+// includes or calls this code; only tests/, the smoke check and the CPU
+// baseline leg of bench.py use it, and there only as the checker.
+//
+// Directed rounding: the reference calls CUDA's per-instruction __dadd_rd,
+// __dmul_ru, __ddiv_rd, ... .  Here every such operation is one SSE2
+// instruction executed under the MXCSR rounding-control field (the register
+// std::fesetround programs for SSE2 arithmetic): each helper loads RD or RU
+// into MXCSR, performs the IEEE-754 operation, and restores round-to-nearest,
+// all inside one asm statement so the compiler cannot move other floating-point
+// work across the mode change.  x86 divsd/sqrtsd/addsd/mulsd are correctly
+// rounded in every mode, so these are the exact IEEE directed results.
+//
+// Build flags: -O2 -mfma -ffp-contract=off -fno-fast-math (see Makefile).
+//
+// Functions whose parity is pinned by tests (tests/test_oracle_*.py):
+//   primitives   exact-rational RD/RU golden vectors (tests/golden/prims_*.npz)
+//   asin_rn      mpmath ulp error and containment of the widened enclosure
+//   lbrsp        closed-form roots of pi r^2 = c (1 - l^2) asin(r / (1 - r))
+//   angles       containment of mpmath point evaluations + monotonicity
+//   criteria     point-sample soundness against mpmath + an exact-rational
+//                interval re-implementation on small cell sets
+//   driver       synthetic trace test of the lambda queue rule
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace cplo {
+
+// ---------------------------------------------------------------------------
+// Directed rounding on x86-64 SSE2 (MXCSR bits 13-14: 00 RN, 01 RD, 10 RU).
+// ---------------------------------------------------------------------------
+namespace mxcsr {
+alignas(4) inline const unsigned rn = 0x1F80u;
+alignas(4) inline const unsigned rd = 0x3F80u;
+alignas(4) inline const unsigned ru = 0x5F80u;
+}  // namespace mxcsr
+
+#define CPLO_DIRECTED_BINOP(NAME, INSN, MODE)                                  \
+  static inline double NAME(double a, double b) {                             \
+    asm("ldmxcsr %[m]\n\t" INSN " %[b], %[a]\n\tldmxcsr %[n]"                 \
+        : [a] "+x"(a)                                                         \
+        : [b] "x"(b), [m] "m"(MODE), [n] "m"(mxcsr::rn));                    \
+    return a;                                                                 \
+  }
+CPLO_DIRECTED_BINOP(add_rd, "addsd", mxcsr::rd)
+CPLO_DIRECTED_BINOP(add_ru, "addsd", mxcsr::ru)
+CPLO_DIRECTED_BINOP(sub_rd, "subsd", mxcsr::rd)
+CPLO_DIRECTED_BINOP(sub_ru, "subsd", mxcsr::ru)
+CPLO_DIRECTED_BINOP(mul_rd, "mulsd", mxcsr::rd)
+CPLO_DIRECTED_BINOP(mul_ru, "mulsd", mxcsr::ru)
+CPLO_DIRECTED_BINOP(div_rd, "divsd", mxcsr::rd)
+CPLO_DIRECTED_BINOP(div_ru, "divsd", mxcsr::ru)
+#undef CPLO_DIRECTED_BINOP
+
+static inline double sqrt_rd(double a) {
+  double r;
+  asm("ldmxcsr %[m]\n\tsqrtsd %[a], %[r]\n\tldmxcsr %[n]"
+      : [r] "=x"(r)
+      : [a] "x"(a), [m] "m"(mxcsr::rd), [n] "m"(mxcsr::rn));
+  return r;
+}
+static inline double sqrt_ru(double a) {
+  double r;
+  asm("ldmxcsr %[m]\n\tsqrtsd %[a], %[r]\n\tldmxcsr %[n]"
+      : [r] "=x"(r)
+      : [a] "x"(a), [m] "m"(mxcsr::ru), [n] "m"(mxcsr::rn));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Constants (reference lane_packing/values.hpp:28-32,
+// algcuda/cuda-utils/cuh/include/algcuda/interval.cuh:358-365).
+// ---------------------------------------------------------------------------
+static const double critical_ratio = 0.5642;
+static const double lb_pi = 3.141592653589793;
+static const double ub_pi = 3.1415926535897936;
+static const double lambda_min = 0.5;
+static const double lambda_max = 0.9900000000000001;
+static const double lb_pi_two = 1.5707963267948966;
+static const double ub_pi_two = 1.5707963267948968;
+
+// ---------------------------------------------------------------------------
+// Device-abort replacement: the reference calls trap() (exit.cuh:28-31); the
+// oracle throws, and the evaluation entry points turn it into a TRAP verdict.
+// ---------------------------------------------------------------------------
+enum TrapCode : int {
+  TRAP_NONE = 0,
+  TRAP_ASIN_DOMAIN = 1,       // interval.cuh:617-621
+  TRAP_OBTUSE_EMPTY_X = 2,    // collision_angles.cu:58-62
+  TRAP_OBTUSE_EMPTY_QUOT = 3, // collision_angles.cu:71-75
+  TRAP_ACUTE_EMPTY_X = 4,     // collision_angles.cu:91-101
+  TRAP_SQRT_DOMAIN = 5,       // interval.cuh:339-343
+};
+struct Trap {
+  int code;
+};
+
+// ---------------------------------------------------------------------------
+// Uncertain<bool> (interval.hpp:32-82): lb = definitely, ub = possibly.
+// ---------------------------------------------------------------------------
+struct UB {
+  bool lb, ub;
+  UB operator!() const { return UB{!ub, !lb}; }
+};
+static inline bool definitely(UB b) { return b.lb; }
+static inline bool possibly(UB b) { return b.ub; }
+
+// ---------------------------------------------------------------------------
+// Interval<double> (interval.hpp:84-242) with the device arithmetic of
+// interval.cuh:30-355.
+// ---------------------------------------------------------------------------
+struct IV {
+  double lb, ub;
+  IV() = default;
+  constexpr IV(double l, double u) : lb(l), ub(u) {}
+  explicit constexpr IV(double v) : lb(v), ub(v) {}
+
+  bool empty() const { return lb > ub; }                       // :212-214
+  void tighten_lb(double v) { if (lb < v) lb = v; }            // :206-210
+  void tighten_ub(double v) { if (v < ub) ub = v; }            // :200-204
+  IV& do_join(const IV& o) {                                   // :190-198
+    if (o.lb < lb) lb = o.lb;
+    if (o.ub > ub) ub = o.ub;
+    return *this;
+  }
+  IV intersect(const IV& o) const {                            // :172-174
+    return IV(lb < o.lb ? o.lb : lb, ub > o.ub ? o.ub : ub);
+  }
+  IV join(const IV& o) const {                                 // :186-188
+    return IV(lb > o.lb ? o.lb : lb, ub < o.ub ? o.ub : ub);
+  }
+  IV abs() const {                                             // :216-232
+    double l = 0.0, u = 0.0;
+    if (ub > 0) u = ub; else l = -ub;
+    if (lb < 0) u = u < -lb ? -lb : u; else l = lb;
+    return IV(l, u);
+  }
+  IV operator-() const { return IV(-ub, -lb); }                // :131
+  IV square() const {                                          // interval.cuh:352-355
+    IV a = abs();
+    return IV(mul_rd(a.lb, a.lb), mul_ru(a.ub, a.ub));
+  }
+};
+
+// + and - (interval.cuh:38-104)
+static inline IV operator+(const IV& a, const IV& b) { return IV(add_rd(a.lb, b.lb), add_ru(a.ub, b.ub)); }
+static inline IV operator+(const IV& a, double b) { return a + IV(b, b); }
+static inline IV operator+(double a, const IV& b) { return b + IV(a, a); }
+static inline IV operator-(const IV& a, const IV& b) { return IV(add_rd(a.lb, -b.ub), add_ru(a.ub, -b.lb)); }
+static inline IV operator-(const IV& a, double b) { return a - IV(b, b); }
+static inline IV operator-(double a, const IV& b) { return IV(a, a) - b; }
+
+// Interval x scalar (interval.cuh:185-197).  For n >= 0 the reference rounds
+// the upper bound with mul_rd; every call on the lane-packing path uses
+// n in {0.25, 0.5, 2.0}, where the product is exact for normal operands, so
+// rounding the upper bound up (as done here) yields the same bits.
+static inline IV operator*(const IV& a, double n) {
+  if (n >= 0) return IV(mul_rd(a.lb, n), mul_ru(a.ub, n));
+  return IV(mul_rd(a.ub, n), mul_ru(a.lb, n));
+}
+static inline IV operator*(double n, const IV& a) { return a * n; }
+
+// Interval x interval, non-branching 8-product form (interval.cuh:171-182).
+static inline IV operator*(const IV& a, const IV& b) {
+  double l1 = mul_rd(a.lb, b.lb), l2 = mul_rd(a.lb, b.ub), l3 = mul_rd(a.ub, b.lb), l4 = mul_rd(a.ub, b.ub);
+  double u1 = mul_ru(a.lb, b.lb), u2 = mul_ru(a.lb, b.ub), u3 = mul_ru(a.ub, b.lb), u4 = mul_ru(a.ub, b.ub);
+  double ll1 = l1 < l2 ? l1 : l2, ll2 = l3 < l4 ? l3 : l4;
+  double uu1 = u1 > u2 ? u1 : u2, uu2 = u3 > u4 ? u3 : u4;
+  return IV(ll1 < ll2 ? ll1 : ll2, uu1 > uu2 ? uu1 : uu2);
+}
+
+// Interval / interval with sign cases (interval.cuh:277-316).
+static inline IV operator/(const IV& a, const IV& o) {
+  double l, u;
+  if (o.lb > 0) {
+    if (a.lb >= 0) { l = div_rd(a.lb, o.ub); u = div_ru(a.ub, o.lb); }
+    else if (a.ub <= 0) { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.ub); }
+    else { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.lb); }
+  } else if (o.ub < 0) {
+    if (a.lb >= 0) { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.lb); }
+    else if (a.ub <= 0) { l = div_rd(a.ub, o.lb); u = div_ru(a.lb, o.ub); }
+    else { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.ub); }
+  } else {
+    u = DBL_MAX * 2.0;  // +infinity (interval.cuh:241-243)
+    l = -u;
+  }
+  return IV(l, u);
+}
+// Interval / scalar (interval.cuh:261-275).
+static inline IV operator/(const IV& a, double n) {
+  if (n > 0) return IV(div_rd(a.lb, n), div_ru(a.ub, n));
+  return IV(div_rd(a.ub, n), div_ru(a.lb, n));
+}
+
+// Interval sqrt (interval.cuh:339-350).
+static inline IV sqrt(const IV& a) {
+  if (a.lb < 0.0) throw Trap{TRAP_SQRT_DOMAIN};
+  return IV(sqrt_rd(a.lb), sqrt_ru(a.ub));
+}
+
+// Comparisons returning Uncertain<bool> (interval.hpp:134-170, 275-289).
+static inline UB operator<(const IV& a, const IV& b) { return UB{a.ub < b.lb, a.lb < b.ub}; }
+static inline UB operator<=(const IV& a, const IV& b) { return UB{a.ub <= b.lb, a.lb <= b.ub}; }
+static inline UB operator>(const IV& a, const IV& b) { return b < a; }
+static inline UB operator>=(const IV& a, const IV& b) { return b <= a; }
+
+// ---------------------------------------------------------------------------
+// asin.  The reference calls CUDA's closed-source ::asin and widens the result
+// by five ulps (interval.cuh:420-438).  Here asin_rn is the project's own
+// round-to-nearest arcsine (argument reduction + degree-12 Chebyshev
+// polynomial, coefficients from circlepacking_amd/csrc/gen_asin_poly.py); the
+// HIP product evaluates the identical sequence of correctly rounded
+// operations, so both sides produce the same bits.  Widening and clamps follow
+// the reference exactly.
+// ---------------------------------------------------------------------------
+static const double asin_poly[13] = {
+    0x1.d72b2bc8155f8p-6,  -0x1.e6aaa8a0a04ccp-7, 0x1.1d189408314eep-6, 0x1.65a9c4dfcf8b2p-8,
+    0x1.52420b04b37bep-7,  0x1.782651caa6547p-7,  0x1.c9cf07674736ap-7, 0x1.1c4d35cf95421p-6,
+    0x1.6e8bb1c8209a2p-6,  0x1.f1c71c1db0623p-6,  0x1.6db6db6e31f13p-5, 0x1.3333333332ecap-4,
+    0x1.5555555555556p-3,
+};
+static const double asin_pio2_hi = 0x1.921fb54442d18p+0;
+static const double asin_pio2_lo = 0x1.1a62633145c07p-54;
+
+static inline double asin_poly_eval(double z) {
+  double p = asin_poly[0];
+  for (int i = 1; i < 13; ++i) p = std::fma(p, z, asin_poly[i]);
+  return p;
+}
+
+static inline double asin_rn(double x) {
+  double ax = std::fabs(x);
+  double r;
+  if (ax <= 0.5) {
+    double z = ax * ax;
+    double p = asin_poly_eval(z);
+    r = std::fma(ax * z, p, ax);
+  } else {
+    double t = 0.5 - 0.5 * ax;           // exact for ax in [0.5, 1]
+    double s = std::sqrt(t);
+    double c = std::fma(-s, s, t) / (s + s);  // sqrt(t) ~= s + c
+    double p = asin_poly_eval(t);
+    double w = std::fma(s * t, p, c);    // asin(s) ~= s + w
+    r = asin_pio2_hi - ((s + s) + (w + w - asin_pio2_lo));
+  }
+  return std::copysign(r, x);
+}
+
+// dasin_rd / dasin_ru (interval.cuh:420-438): 5 ulp steps outward, clamps.
+static inline double dasin_rd(double x) {
+  double y = asin_rn(x);
+  for (int i = 0; i < 5; ++i) y = std::nextafter(y, -2.0);
+  if (y < -ub_pi_two) y = -ub_pi_two;
+  if (x >= 0.0 && y < 0.0) y = 0.0;
+  return y;
+}
+static inline double dasin_ru(double x) {
+  double y = asin_rn(x);
+  for (int i = 0; i < 5; ++i) y = std::nextafter(y, 2.0);
+  if (y > ub_pi_two) y = ub_pi_two;
+  if (x <= 0.0 && y > 0.0) y = 0.0;
+  return y;
+}
+// Interval asin (interval.cuh:617-624).
+static inline IV asin(const IV& x) {
+  if (x.lb < -1.0 || x.ub > 1.0) throw Trap{TRAP_ASIN_DOMAIN};
+  return IV(dasin_rd(x.lb), dasin_ru(x.ub));
+}
+
+// ---------------------------------------------------------------------------
+// get_subinterval (subintervals.hpp:31-42).  nvcc contracts lb + diff*i into a
+// fused multiply-add by default; the restatement writes the fma explicitly.
+// ---------------------------------------------------------------------------
+static inline IV get_subinterval(IV iv, int i, int n) {
+  double lb = iv.lb, ub = iv.ub;
+  double diff = ub - lb;
+  diff /= n;
+  double l = std::fma(diff, (double)i, lb);
+  double u = std::fma(diff, (double)(i + 1), lb);
+  if (i + 1 == n) u = ub;
+  return IV(l, u);
+}
+
+// ---------------------------------------------------------------------------
+// Lane geometry (lane_packing/collision_angles.cu, lower_bound_radius_single_placement.cu)
+// ---------------------------------------------------------------------------
+// collision_angles.cu:29-31
+static inline UB is_obtuse(IV hypot, IV s1, IV s2) { return hypot.square() > s1.square() + s2.square(); }
+
+// collision_angles.cu:37-117
+static inline IV compute_collision_angle(IV lc, IV sc, IV ab) {
+  lc.tighten_lb(sc.lb);
+  sc.tighten_ub(lc.ub);
+  IV result(0.0, -1.0);
+  UB obt = is_obtuse(lc, sc, ab);
+  IV lc_sq = lc.square();
+  IV sc_sq = sc.square();
+  IV add_term = (lc_sq - sc_sq) / ab;
+  add_term.tighten_lb(0.0);
+
+  if (possibly(obt)) {
+    IV lx = 0.5 * (add_term + ab);
+    IV sx = 0.5 * (add_term - ab);
+    sx.tighten_lb(0.0);
+    sx.tighten_ub(sc.ub);
+    lx.tighten_ub(lc.ub);
+    if (sx.empty() || lx.empty()) {
+      if (definitely(obt)) throw Trap{TRAP_OBTUSE_EMPTY_X};
+    } else {
+      IV lquot = lx / lc;
+      lquot.tighten_ub(1.0);
+      IV squot = sx / sc;
+      squot.tighten_ub(1.0);
+      if (lquot.empty() || squot.empty()) {
+        if (definitely(obt)) throw Trap{TRAP_OBTUSE_EMPTY_QUOT};
+      } else {
+        IV alpha = asin(lquot);
+        IV beta = asin(squot);
+        result = alpha - beta;
+        result.tighten_lb(0.0);
+      }
+    }
+  }
+
+  if (!definitely(obt)) {
+    IV lx = 0.5 * (ab + add_term);
+    IV sx = 0.5 * (ab - add_term);
+    sx.tighten_lb(0.0);
+    if (sx.empty()) {
+      if (!possibly(obt)) throw Trap{TRAP_ACUTE_EMPTY_X};
+    } else {
+      IV alpha = asin(lx / lc);
+      IV beta = asin(sx / sc);
+      beta.tighten_lb(0.0);
+      IV r = alpha + beta;
+      if (result.empty()) result = r;
+      else result.do_join(r);
+    }
+  }
+  return result;
+}
+
+// collision_angles.cu:119-142
+static inline IV collision_angle_inner_inner(IV la, IV ra, IV rb) {
+  double lb, ub;
+  {
+    IV lc = la.lb + IV(ra.ub, ra.ub);
+    IV sc = la.lb + IV(rb.ub, rb.ub);
+    IV ab = IV(ra.ub, ra.ub) + rb.ub;
+    ub = compute_collision_angle(lc, sc, ab).ub;
+  }
+  {
+    IV lc = la.ub + IV(ra.lb, ra.lb);
+    IV sc = la.ub + IV(rb.lb, rb.lb);
+    IV ab = IV(ra.lb, ra.lb) + rb.lb;
+    if (ab.lb <= 0.0) lb = 0.0;
+    else lb = compute_collision_angle(lc, sc, ab).lb;
+  }
+  return IV(lb, ub);
+}
+
+// collision_angles.cu:144-159
+static inline IV do_collision_angle_outer_inner(IV la, IV ra, IV rb) {
+  IV ca = 1.0 - ra;
+  IV cb = la + rb;
+  IV ab = ra + rb;
+  UB ca_longer = (cb <= ca);
+  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab);
+  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab);
+  IV result = compute_collision_angle(ca, cb, ab);
+  result.do_join(compute_collision_angle(cb, ca, ab));
+  return result;
+}
+
+// collision_angles.cu:161-174
+static inline IV collision_angle_outer_inner(IV la, IV ra, IV rb) {
+  double lb, ub;
+  if (possibly(2.0 * (ra + rb) <= 1.0 - la)) lb = 0.0;
+  else lb = do_collision_angle_outer_inner(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb)).lb;
+  ub = do_collision_angle_outer_inner(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub)).ub;
+  return IV(lb, ub);
+}
+
+// collision_angles.cu:176-191
+static inline IV do_collision_angle_inner_outer(IV la, IV ra, IV rb) {
+  IV ca = la + ra;
+  IV cb = 1.0 - rb;
+  IV ab = ra + rb;
+  UB ca_longer = (cb <= ca);
+  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab);
+  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab);
+  IV result = compute_collision_angle(ca, cb, ab);
+  result.do_join(compute_collision_angle(cb, ca, ab));
+  return result;
+}
+
+// collision_angles.cu:193-205
+static inline IV collision_angle_inner_outer(IV la, IV ra, IV rb) {
+  double lb, ub;
+  if (possibly(2.0 * (ra + rb) <= 1.0 - la)) lb = 0.0;
+  else lb = do_collision_angle_inner_outer(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb)).lb;
+  ub = do_collision_angle_inner_outer(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub)).ub;
+  return IV(lb, ub);
+}
+
+// collision_angles.cu:207-231
+static inline IV collision_angle_outer_outer(IV ra, IV rb) {
+  double lb, ub;
+  {
+    IV lc = IV(1.0, 1.0) - rb.ub;
+    IV sc = IV(1.0, 1.0) - ra.ub;
+    IV ab = IV(ra.ub, ra.ub) + rb.ub;
+    ub = compute_collision_angle(lc, sc, ab).ub;
+  }
+  {
+    IV lc = IV(1.0, 1.0) - rb.lb;
+    IV sc = IV(1.0, 1.0) - ra.lb;
+    IV ab = IV(ra.lb, ra.lb) + rb.lb;
+    if (ab.lb <= 0.0) lb = 0.0;
+    else lb = compute_collision_angle(lc, sc, ab).lb;
+  }
+  return IV(lb, ub);
+}
+
+// collision_angles.cu:233-238
+static inline IV semicircle_angle_outer(IV r) {
+  return IV(dasin_rd(div_rd(r.lb, add_ru(1.0, -r.lb))),
+            dasin_ru(div_ru(r.ub, add_rd(1.0, -r.ub))));
+}
+// collision_angles.cu:240-246
+static inline IV semicircle_angle_inner(IV la, IV r) {
+  return IV(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))),
+            dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
+}
+
+// lower_bound_radius_single_placement.cu:26-61
+static inline double lower_bound_radius_single_placement(double lb_lambda) {
+  const double ub_rhs_factor = div_ru(critical_ratio, lb_pi);
+  const double ub_rhs = mul_ru(ub_rhs_factor, add_ru(1.0, -mul_rd(lb_lambda, lb_lambda)));
+  double ub = 0.25;
+  double lb = 5.0e-324;
+  for (;;) {
+    double mid = 0.5 * (lb + ub);
+    if (mid <= lb || mid >= ub) return ub;
+    double r_sq = mul_rd(mid, mid);
+    double ub_quot = div_ru(mid, add_rd(1.0, -mid));
+    double ub_asin = dasin_ru(ub_quot);
+    double lb_lhs = div_rd(r_sq, ub_asin);
+    if (lb_lhs >= ub_rhs) ub = mid;
+    else lb = mid;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Families, verdicts and the per-cell kernel body.
+// ---------------------------------------------------------------------------
+enum Family : int {
+  START = 0,         // start_configuration.cu:114-167
+  START_END_2D = 1,  // start_configuration.cu:190-215
+  START_END_3D = 2,  // start_configuration.cu:267-298
+  MIDDLE = 3,        // middle_configuration.cu:156-205
+  END_2D = 4,        // end_configuration.cu:103-146
+  END_3D = 5,        // end_configuration.cu:212-268
+  NUM_FAMILIES = 6,
+};
+static inline bool family_is_3d(int f) { return f == START || f == START_END_3D || f == MIDDLE || f == END_3D; }
+
+enum Verdict : int { PROVEN = 0, CRITICAL = 1, NOFIT = 2, EMPTY = 3, OOD = 4, TRAP = 5 };
+// Verdict byte: bits 0-2 verdict, bit 3 NOFIT flag (start family: the
+// reference only prints and keeps evaluating), bits 4-7 detail: the trap code
+// for TRAP, the 1-based failing criterion for CRITICAL.
+static inline uint8_t vbyte(int v, int nofit_flag, int detail) {
+  return (uint8_t)(v | (nofit_flag ? 8 : 0) | (detail << 4));
+}
+
+// Range functions (a11).  Shared r1 rule: start_configuration.cu:58-66,
+// middle_configuration.cu:66-74, end_configuration.cu:55-63.
+static inline IV compute_r1_range(IV la, double lbrsp_of_la_lb) {
+  double ub_by_single_placement = lbrsp_of_la_lb;
+  double ub_by_lane_width = (0.5 * (1.0 - la)).ub;
+  double lb_by_lane_width = (0.25 * (1.0 - la)).lb;
+  return IV(lb_by_lane_width, ub_by_lane_width < ub_by_single_placement ? ub_by_lane_width : ub_by_single_placement);
+}
+// start_configuration.cu:68-71, middle_configuration.cu:76-79
+static inline IV r2_range_quarter(IV la, IV r1) { return IV((0.25 * (1.0 - la)).lb, r1.ub); }
+// start_configuration.cu:74-77
+static inline IV r2_range_2d_start(IV la, IV r1) { return IV((0.5 * (1.0 - la - 2.0 * r1)).lb, r1.ub); }
+// start_configuration.cu:79-82, middle_configuration.cu:81-84
+static inline IV r3_range_quarter(IV la, IV /*r1*/, IV r2) { return IV((0.25 * (1.0 - la)).lb, r2.ub); }
+// start_configuration.cu:84-87
+static inline IV r3_range_3d_start(IV la, IV /*r1*/, IV r2) { return IV((0.5 * (1.0 - la - 2.0 * r2)).lb, r2.ub); }
+// end_configuration.cu:65-69
+static inline IV r2_range_end(IV la, IV r1) {
+  double l = mul_rd(0.5, add_rd(1.0, -add_ru(la.ub, mul_ru(2.0, r1.ub))));
+  if (l < 0.0) l = 0.0;
+  return IV(l, r1.ub);
+}
+// end_configuration.cu:148-152
+static inline IV r3_range_end_3d(IV la, IV /*r1*/, IV r2) {
+  double l = mul_rd(0.5, add_rd(1.0, -add_ru(la.ub, mul_ru(2.0, r2.ub))));
+  if (l < 0.0) l = 0.0;
+  return IV(l, r2.ub);
+}
+static inline IV r2_range_for(int f, IV la, IV r1) {
+  switch (f) {
+    case START_END_2D: return r2_range_2d_start(la, r1);
+    case END_2D: case END_3D: return r2_range_end(la, r1);
+    default: return r2_range_quarter(la, r1);
+  }
+}
+static inline IV r3_range_for(int f, IV la, IV r1, IV r2) {
+  switch (f) {
+    case START_END_3D: return r3_range_3d_start(la, r1, r2);
+    case END_3D: return r3_range_end_3d(la, r1, r2);
+    default: return r3_range_quarter(la, r1, r2);
+  }
+}
+
+// ---- start configuration criteria (start_configuration.cu:93-112) ----
+static inline bool handle_case_1(IV la, IV r1, IV r2, IV a12) {
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  IV a = semicircle_angle_outer(r1) + a12;
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  double lb_disk_area = add_rd(lb_area1, mul_rd(0.5, lb_area2));
+  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+}
+static inline bool handle_case_2(IV la, IV r1, IV r2, IV r3, IV a13) {
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
+  IV a = semicircle_angle_outer(r1) + a13;
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  double lb_disk_area = add_rd(add_rd(lb_area1, lb_area2), mul_rd(0.5, lb_area3));
+  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+}
+
+// start_configuration.cu:169-188
+static inline bool handle_2d_case(IV la, IV r1, IV r2) {
+  IV a1 = semicircle_angle_outer(r1);
+  IV a2 = semicircle_angle_inner(la, r2);
+  IV a12 = collision_angle_outer_inner(la, r1, r2);
+  IV total_angle = a1 + a2 + a12;
+  IV reuse_angle = a12 + a2 - a1;
+  reuse_angle.tighten_lb(0.0);
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, total_angle.ub));
+  IV rho = 1.0 - (la + 2.0 * r2).square();
+  rho.tighten_lb(0.0);
+  double lb_alpha_reusable = reuse_angle.lb;
+  double lb_area_reusable = mul_rd(mul_rd(0.5, lb_alpha_reusable), rho.lb);
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_disk_area = add_rd(lb_area2, lb_area1);
+  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+}
+
+// start_configuration.cu:227-265 (keeps the reference's outer_outer(r1, r2) at :232)
+static inline bool handle_3d_case(IV la, IV r1, IV r2, IV r3) {
+  IV a1 = semicircle_angle_outer(r1);
+  IV a2 = semicircle_angle_inner(la, r2);
+  IV a3 = semicircle_angle_outer(r3);
+  IV a12 = collision_angle_outer_inner(la, r1, r2);
+  IV a13 = collision_angle_outer_outer(r1, r2);
+  IV a23 = collision_angle_inner_outer(la, r2, r3);
+  IV b13_123 = a12 + a23;
+  IV b13(a13.lb < b13_123.lb ? b13_123.lb : a13.lb, a13.ub < b13_123.ub ? b13_123.ub : a13.ub);
+  IV a123 = a1 + b13 + a3;
+  IV a122 = a1 + a12 + a2;
+  IV a(a123.lb < a122.lb ? a122.lb : a123.lb, a123.ub < a122.ub ? a122.ub : a123.ub);
+  double lb_reusable_angle = (a - a122).lb;
+  if (lb_reusable_angle < 0.0) lb_reusable_angle = 0.0;
+  double lb_outer_rad = add_rd(1.0, -mul_ru(2.0, r3.ub));
+  double lb_outer_rad_sq = mul_rd(lb_outer_rad, lb_outer_rad);
+  double ub_inner_rad = la.ub;
+  double ub_inner_rad_sq = mul_ru(ub_inner_rad, ub_inner_rad);
+  double lb_area_reused = mul_rd(mul_rd(0.5, lb_reusable_angle), add_rd(lb_outer_rad_sq, -ub_inner_rad_sq));
+  if (lb_area_reused < 0.0) lb_area_reused = 0.0;
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  ub_area_used = add_ru(ub_area_used, -lb_area_reused);
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
+  double lb_disk_area = add_rd(lb_area1, add_rd(lb_area2, lb_area3));
+  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+}
+
+// ---- middle configuration (middle_configuration.cu:86-154) ----
+static inline bool middle_area_checks(IV la, IV r1, IV r2, IV r3, IV c12, IV c13, IV c23) {
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  UB case_a = (c12 + c23 >= c13);
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  if (possibly(case_a)) {
+    double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c12.ub));
+    double lb_disk_area = mul_rd(0.5, add_rd(lb_area1, lb_area2));
+    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+  }
+  if (possibly(!case_a)) {
+    double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c13.ub));
+    double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
+    double lb_disk_area = add_rd(mul_rd(0.5, add_rd(lb_area1, lb_area3)), lb_area2);
+    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+  }
+  return true;
+}
+static inline bool handle_case_1_1(IV la, IV r1, IV r2, IV r3) {
+  IV c12 = collision_angle_inner_outer(la, r1, r2);
+  IV c13 = collision_angle_inner_inner(la, r1, r3);
+  IV c23 = collision_angle_outer_inner(la, r2, r3);
+  return middle_area_checks(la, r1, r2, r3, c12, c13, c23);
+}
+static inline bool handle_case_2_1(IV la, IV r1, IV r2, IV r3) {
+  IV c12 = collision_angle_outer_inner(la, r1, r2);
+  IV c13 = collision_angle_outer_outer(r1, r3);
+  IV c23 = collision_angle_inner_outer(la, r2, r3);
+  return middle_area_checks(la, r1, r2, r3, c12, c13, c23);
+}
+
+// ---- end configuration, 2 disks (end_configuration.cu:71-101) ----
+static inline bool handle_inner_outer_2d(IV la, IV r1, IV r2) {
+  IV a = collision_angle_inner_outer(la, r1, r2) + semicircle_angle_outer(r2);
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  IV rho = (1.0 - 2.0 * r2).square() - la.square();
+  rho.tighten_lb(0.0);
+  double lb_alpha_reusable = (a - semicircle_angle_inner(la, r1)).lb;
+  double lb_area_reusable = mul_rd(mul_rd(0.5, lb_alpha_reusable), rho.lb);
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_disk_area = add_rd(lb_area2, mul_rd(0.5, lb_area1));
+  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+}
+static inline bool handle_outer_inner_2d(IV la, IV r1, IV r2) {
+  IV a = collision_angle_outer_inner(la, r1, r2) + semicircle_angle_inner(la, r2);
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  IV rho = 1.0 - (la + 2.0 * r2).square();
+  rho.tighten_lb(0.0);
+  double lb_alpha_reusable = (a - semicircle_angle_outer(r1)).lb;
+  double lb_area_reusable = mul_rd(mul_rd(0.5, lb_alpha_reusable), rho.lb);
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_disk_area = add_rd(lb_area2, mul_rd(0.5, lb_area1));
+  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+}
+
+// ---- end configuration, 3 disks (end_configuration.cu:154-210) ----
+static inline bool handle_r1_inner_3d(IV la, IV r1, IV r2, IV r3) {
+  IV a12 = collision_angle_inner_outer(la, r1, r2);
+  IV a23 = collision_angle_outer_inner(la, r2, r3);
+  IV a13 = collision_angle_inner_inner(la, r1, r3);
+  if (definitely(a12 + a23 >= a13)) return true;
+  IV a3 = semicircle_angle_inner(la, r3);
+  IV a = a13 + a3;
+  IV a2 = semicircle_angle_outer(r2);
+  IV reusable_alpha = a - (a12 + a2);
+  IV rho = 1.0 - (la + 2.0 * r3).square();
+  rho.tighten_lb(0.0);
+  double lb_area_reusable = mul_rd(mul_rd(0.5, reusable_alpha.lb), rho.lb);
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
+  double lb_disk_area = add_rd(add_rd(lb_area2, lb_area3), mul_rd(0.5, lb_area1));
+  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+}
+static inline bool handle_r1_outer_3d(IV la, IV r1, IV r2, IV r3) {
+  IV a12 = collision_angle_outer_inner(la, r1, r2);
+  IV a23 = collision_angle_inner_outer(la, r2, r3);
+  IV a13 = collision_angle_outer_outer(r1, r3);
+  if (definitely(a12 + a23 >= a13)) return true;
+  IV a3 = semicircle_angle_outer(r3);
+  IV a = a13 + a3;
+  IV a2 = semicircle_angle_inner(la, r2);
+  IV reusable_alpha = a - (a12 + a2);
+  IV rho = (1.0 - 2.0 * r3).square() - la.square();
+  rho.tighten_lb(0.0);
+  double lb_area_reusable = mul_rd(mul_rd(0.5, reusable_alpha.lb), rho.lb);
+  double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
+  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
+  double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
+  double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
+  double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
+  double lb_disk_area = add_rd(add_rd(lb_area2, lb_area3), mul_rd(0.5, lb_area1));
+  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+}
+
+// Fit check shared by start (print only), middle and end (trap):
+// start_configuration.cu:131, middle_configuration.cu:173, end_configuration.cu:115,236.
+static inline bool possibly_no_fit(IV la, IV r1) { return mul_ru(2.0, r1.ub) >= add_rd(1.0, -la.ub); }
+
+// One cell of family f, from the kernel bodies: r1_ and r2_ are the grid
+// subintervals before tightening, r3 the innermost subinterval (unused for 2d
+// families).  Returns the verdict byte.
+static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3) {
+  IV r2 = r2_;
+  if (family_is_3d(f)) r2.tighten_lb(r3.lb);
+  IV r1 = r1_;
+  r1.tighten_lb(r2.lb);
+  if (r1.empty() || r2.empty()) return vbyte(EMPTY, 0, 0);
+  int nofit_flag = 0;
+  if (f == START || f == MIDDLE || f == END_2D || f == END_3D) {
+    if (possibly_no_fit(la, r1)) {
+      if (f != START) return vbyte(NOFIT, 0, 0);
+      nofit_flag = 1;
+    }
+  }
+  try {
+    switch (f) {
+      case START: {  // start_configuration.cu:142-162
+        IV a12 = collision_angle_outer_inner(la, r1, r2);
+        IV a13 = collision_angle_outer_outer(r1, r3);
+        IV a23 = collision_angle_inner_outer(la, r2, r3);
+        UB case1 = (a12 + a23 > a13);
+        if (possibly(case1) && !handle_case_1(la, r1, r2, a12)) return vbyte(CRITICAL, nofit_flag, 1);
+        if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13)) return vbyte(CRITICAL, nofit_flag, 2);
+        return vbyte(PROVEN, nofit_flag, 0);
+      }
+      case START_END_2D:
+        return handle_2d_case(la, r1, r2) ? vbyte(PROVEN, 0, 0) : vbyte(CRITICAL, 0, 1);
+      case START_END_3D:
+        return handle_3d_case(la, r1, r2, r3) ? vbyte(PROVEN, 0, 0) : vbyte(CRITICAL, 0, 1);
+      case MIDDLE:
+        if (!handle_case_1_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 1);
+        if (!handle_case_2_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
+        return vbyte(PROVEN, 0, 0);
+      case END_2D:
+        if (!handle_inner_outer_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 1);
+        if (!handle_outer_inner_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 2);
+        return vbyte(PROVEN, 0, 0);
+      case END_3D:
+        if (!handle_r1_inner_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 1);
+        if (!handle_r1_outer_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
+        return vbyte(PROVEN, 0, 0);
+    }
+  } catch (const Trap& t) {
+    return vbyte(TRAP, nofit_flag, t.code);
+  }
+  return vbyte(TRAP, 0, 15);
+}
+
+// r3 from a position interval p in [0,1] over the admissible r3 range; for a
+// root cell p = [k/n, (k+1)/n] this equals get_subinterval(range, k, n).
+static inline IV r3_from_position(IV range, IV p) {
+  double w = range.ub - range.lb;
+  double l = std::fma(w, p.lb, range.lb);
+  double u = (p.ub == 1.0) ? range.ub : std::fma(w, p.ub, range.lb);
+  return IV(l, u);
+}
+
+// A synthetic cell (la, r1, r2, p3) for family f.  With clip, each radius is
+// first intersected with f's admissible range (OOD if that is empty);
+// without clip the box is evaluated as given.  lbrsp is
+// lower_bound_radius_single_placement(la.lb).
+static inline uint8_t eval_synthetic(int f, IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp) {
+  IV r1c = r1, r2c = r2;
+  if (clip) {
+    r1c = r1.intersect(compute_r1_range(la, lbrsp));
+    if (r1c.empty()) return vbyte(OOD, 0, 0);
+    r2c = r2.intersect(r2_range_for(f, la, r1c));
+    if (r2c.empty()) return vbyte(OOD, 0, 0);
+  }
+  IV r3(0.0, 0.0);
+  if (family_is_3d(f)) {
+    IV range = r3_range_for(f, la, r1c, r2c);
+    if (range.empty()) return vbyte(OOD, 0, 0);
+    r3 = r3_from_position(range, p3);
+  }
+  return eval_kernel_body(f, la, r1c, r2c, r3);
+}
+
+// Atomic_ordered_double's order-preserving map (atomic_float.hpp:56-66).
+static inline uint64_t ordered_from_double(double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  uint64_t mask = (0 - (b >> 63)) | (uint64_t(1) << 63);
+  return b ^ mask;
+}
+static inline double double_from_ordered(uint64_t v) {
+  uint64_t mask = ((v >> 63) - 1) | (uint64_t(1) << 63);
+  uint64_t b = v ^ mask;
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+
+}  // namespace cplo

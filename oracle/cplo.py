@@ -1,0 +1,113 @@
+"""ctypes binding of the CPU oracle (liboracle_cpl.so) — TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this module,
+as the checker; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from circlepacking_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_cpl.so")
+_lib = None
+_P_D = ctypes.POINTER(ctypes.c_double)
+_P_U8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.cplo_prim.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.c_size_t]
+        L.cplo_geom.argtypes = [ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
+        L.cplo_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
+        L.cplo_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8,
+                                      ctypes.c_int]
+        L.cplo_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
+        L.cplo_first_problematic.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(ctypes.c_int32), _P_D, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_P_D)
+
+
+def prim(op, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty_like(a)
+    bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
+    rc = lib().cplo_prim(abi.PRIM_OPS[op], _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size)
+    assert rc == 0
+    return out
+
+
+GEOM = {"inner_inner": 0, "outer_inner": 1, "inner_outer": 2, "outer_outer": 3,
+        "semicircle_outer": 4, "semicircle_inner": 5, "compute": 6}
+
+
+def geom(fn, recs):
+    """recs: (n, 6) = la.lb, la.ub, ra.lb, ra.ub, rb.lb, rb.ub -> ((n, 2) enclosure, (n,) trap codes)."""
+    recs = np.ascontiguousarray(recs, dtype=np.float64)
+    n = recs.shape[0]
+    out = np.empty((n, 2), dtype=np.float64)
+    trap = np.zeros(n, dtype=np.int32)
+    rc = lib().cplo_geom(GEOM[fn], _dptr(recs), _dptr(out), trap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+    assert rc == 0
+    return out, trap
+
+
+def eval_kernel_body(family, cells):
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    out = np.empty(cells.shape[0], dtype=np.uint8)
+    assert lib().cplo_eval_kernel_body(family, _dptr(cells), cells.shape[0], out.ctypes.data_as(_P_U8)) == 0
+    return out
+
+
+def eval_cells(family, soa, clip=True, nthreads=0):
+    soa = np.ascontiguousarray(soa, dtype=np.float64)
+    n = soa.shape[1]
+    out = np.empty(n, dtype=np.uint8)
+    ptrs = (_P_D * 8)(*[_dptr(soa[k]) for k in range(8)])
+    assert lib().cplo_eval_cells(family, int(clip), ptrs, n, out.ctypes.data_as(_P_U8), nthreads) == 0
+    return out
+
+
+def refine_run(cfg, want_flags=True, root_begin=0, root_end=0, root_stride=1, nthreads=0):
+    st = abi.RefineStats()
+    n_roots = cfg.n_roots()
+    end = root_end or n_roots
+    count = max(0, (end - root_begin + root_stride - 1) // root_stride)
+    flags = None
+    fp = None
+    if want_flags:
+        flags = np.zeros((count, len(cfg.families())), dtype=np.uint8)
+        fp = flags.ctypes.data_as(_P_U8)
+    rc = lib().cplo_refine_run(ctypes.byref(cfg), ctypes.byref(st), fp, root_begin, root_end, root_stride, nthreads)
+    assert rc == 0
+    return st, flags
+
+
+def first_problematic(family, outer_la, subdiv, nthreads=0):
+    """-> (rc, first, evaluated, trap_code)"""
+    first = ctypes.c_double()
+    ev = ctypes.c_uint64()
+    trap = ctypes.c_int()
+    sd = (ctypes.c_int32 * 4)(*subdiv)
+    rc = lib().cplo_first_problematic(family, outer_la[0], outer_la[1], sd, ctypes.byref(first), ctypes.byref(ev),
+                                      ctypes.byref(trap), nthreads)
+    return rc, first.value, ev.value, trap.value

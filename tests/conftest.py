@@ -1,0 +1,26 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def cplo():
+    from oracle import cplo as mod
+    mod.lib()
+    return mod
+
+
+@pytest.fixture(scope="session")
+def cpl():
+    import circlepacking_amd as mod
+    mod.lib()  # raises if the HIP library is missing: no silent fallback
+    return mod
