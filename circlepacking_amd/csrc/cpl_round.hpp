@@ -90,7 +90,7 @@ __device__ __forceinline__ void mul_pair(double al, double bl, double au, double
 __device__ __forceinline__ uint64_t dbits(double d) { return __double_as_longlong(d); }
 __device__ __forceinline__ double dfrom(uint64_t b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ int dexp_field(double d) { return (int)((dbits(d) >> 52) & 0x7FF); }
-// 2^k for -1022 <= k <= 1023
+// 2^k for -1022 <= k <= 1023 (normal powers of two only)
 __device__ __forceinline__ double pow2(int k) { return dfrom((uint64_t)(k + 1023) << 52); }
 
 // ---- division ---------------------------------------------------------------
@@ -156,8 +156,8 @@ __device__ __attribute__((noinline)) double div_slow(double a, double b) {
   if (dexp_field(b) == 0) { b *= 0x1p+600; shift += 600; }
   if (dexp_field(a) == 0) { a *= 0x1p+600; shift -= 600; }
   int eb = dexp_field(b) - 1023, ea = dexp_field(a) - 1023;
-  b *= pow2(-eb);
-  a *= pow2(-ea);
+  b = __builtin_ldexp(b, -eb);  // exact: the results are normal
+  a = __builtin_ldexp(a, -ea);
   shift += ea - eb;
   double qs = a / b;  // |qs| in (1/2, 2)
   double ds = UP ? div_fix_ru(qs, b, a, 0.5) : div_fix_rd(qs, b, a, 0.5);
@@ -170,10 +170,10 @@ template <bool UP>
 __device__ __forceinline__ double div_dir(double a, double b) {
   double q = a / b;  // IEEE round-to-nearest
   if (__builtin_expect(exp_comfortable(a) & exp_comfortable(b) & exp_comfortable(q), 1)) {
+    // a/b = (-a)/(-b): normalise the divisor positive, the quotient is unchanged
     double sa = b < 0.0 ? -a : a;
     double sb = b < 0.0 ? -b : b;
-    double sq = b < 0.0 ? -q : q;
-    return div_core<UP>(sa, sb, sq);
+    return div_core<UP>(sa, sb, q);
   }
   return div_slow<UP>(a, b);
 }
