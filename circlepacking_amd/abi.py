@@ -69,6 +69,8 @@ class RefineStats(ctypes.Structure):
         ("verdicts", (ctypes.c_uint64 * 8) * MAX_ROUNDS),
         ("digest", ctypes.c_uint64 * MAX_ROUNDS),
         ("verdict_digest", ctypes.c_uint64 * MAX_ROUNDS),
+        ("family_verdicts", ((ctypes.c_uint64 * 8) * 6) * MAX_ROUNDS),
+        ("launch_seconds", (ctypes.c_double * 6) * MAX_ROUNDS),
         ("total_evaluations", ctypes.c_uint64),
         ("certified", ctypes.c_uint64),
         ("unresolved", ctypes.c_uint64),
@@ -85,6 +87,7 @@ class RefineStats(ctypes.Structure):
             "verdicts": [list(self.verdicts[d][:6]) for d in range(r)],
             "digest": list(self.digest[:r]),
             "verdict_digest": list(self.verdict_digest[:r]),
+            "family_verdicts": [[list(self.family_verdicts[d][f][:6]) for f in range(6)] for d in range(r)],
             "total_evaluations": self.total_evaluations,
             "certified": self.certified,
             "unresolved": self.unresolved,

@@ -53,10 +53,22 @@ def build_hip(verbose=False):
     return LIB
 
 
+PROVER = os.path.join(HERE, "cpl_prover")
+
+
 def build_driver(verbose=False):
-    """C++ prover driver (added with the driver sources)."""
-    return None
+    """The C++ prover executable (prover.cpp), linked against libcpl_hip.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", PROVER, os.path.join(CSRC, "prover.cpp"),
+           "-I", os.path.join(ROOT, "include"), "-L", HERE, "-lcpl_hip", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed ({res.returncode}):\n{res.stderr}")
+    return PROVER
 
 
 if __name__ == "__main__":
     print(build_hip(verbose=True))
+    print(build_driver(verbose=True))

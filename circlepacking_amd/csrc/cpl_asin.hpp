@@ -41,7 +41,7 @@ __device__ __attribute__((noinline)) double asin_rn(double x) {
   } else {
     double t = 0.5 - 0.5 * ax;  // exact for ax in [1/2, 1]
     double s = __builtin_sqrt(t);
-    double c = __builtin_fma(-s, s, t) / (s + s);
+    double c = t > 0.0 ? __builtin_fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
     double p = asin_poly_eval(t);
     double w = __builtin_fma(s * t, p, c);
     r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));

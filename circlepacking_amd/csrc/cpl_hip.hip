@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void k_round(RefineParams P, Worklist in, uint
       }
     }
   }
-  RoundStats* S = P.stats + P.depth;
+  RoundStats* S = P.stats + P.depth * CPL_NUM_FAMILIES + F;
   wave_add(&S->evaluations, c_eval);
 #pragma unroll
   for (int v = 0; v < 8; ++v) wave_add(&S->verdicts[v], c_v[v]);
@@ -563,7 +563,8 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   if ((rc = g.la_table.ensure(la_entries * 3 * sizeof(double)))) return rc;
   const uint64_t flag_bytes = ((n_roots * nf + 3) / 4) * 4;
   if ((rc = g.root_flags.ensure(flag_bytes))) return rc;
-  if ((rc = g.stats.ensure(CPL_MAX_ROUNDS * sizeof(RoundStats)))) return rc;
+  const size_t stats_bytes = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats);
+  if ((rc = g.stats.ensure(stats_bytes))) return rc;
   if ((rc = g.counters.ensure(2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
   const char* cap_env = getenv("CPL_WORKLIST_CAP");
   uint64_t cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
@@ -576,9 +577,10 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   std::vector<hipEvent_t> kev;
+  std::vector<int> kslot;
 
   HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, flag_bytes, g.stream));
-  HIP_TRY(hipMemsetAsync(g.stats.p, 0, CPL_MAX_ROUNDS * sizeof(RoundStats), g.stream));
+  HIP_TRY(hipMemsetAsync(g.stats.p, 0, stats_bytes, g.stream));
   HIP_TRY(hipMemsetAsync(g.counters.p, 0, 2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
   HIP_TRY(hipEventRecord(ev0, g.stream));
   hipLaunchKernelGGL(k_la_table, dim3((unsigned)((la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
@@ -631,6 +633,7 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
       HIP_TRY(hipEventRecord(b, g.stream));
       kev.push_back(a);
       kev.push_back(b);
+      kslot.push_back(d * CPL_NUM_FAMILIES + fams[s]);
     }
     if (!any) break;
     // next-round sizes (one small copy per round) and reset of the consumed counters
@@ -651,19 +654,25 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   HIP_TRY(hipEventRecord(ev1, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
 
-  std::vector<RoundStats> rs(CPL_MAX_ROUNDS);
-  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, CPL_MAX_ROUNDS * sizeof(RoundStats), hipMemcpyDeviceToHost));
+  std::vector<RoundStats> rs(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES);
+  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, stats_bytes, hipMemcpyDeviceToHost));
   if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, n_roots * nf, hipMemcpyDeviceToHost));
   std::memset(stats, 0, sizeof(*stats));
   for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
-    stats->evaluations[d] = rs[d].evaluations;
-    for (int v = 0; v < 8; ++v) stats->verdicts[d][v] = rs[d].verdicts[v];
-    stats->digest[d] = rs[d].digest;
-    stats->verdict_digest[d] = rs[d].vdigest;
-    stats->certified += rs[d].certified;
-    stats->unresolved += rs[d].unresolved;
-    stats->total_evaluations += rs[d].evaluations;
-    if (rs[d].evaluations) stats->rounds = d + 1;
+    for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
+      const RoundStats& r = rs[d * CPL_NUM_FAMILIES + f];
+      stats->evaluations[d] += r.evaluations;
+      for (int v = 0; v < 8; ++v) {
+        stats->verdicts[d][v] += r.verdicts[v];
+        stats->family_verdicts[d][f][v] = r.verdicts[v];
+      }
+      stats->digest[d] += r.digest;
+      stats->verdict_digest[d] += r.vdigest;
+      stats->certified += r.certified;
+      stats->unresolved += r.unresolved;
+      stats->total_evaluations += r.evaluations;
+    }
+    if (stats->evaluations[d]) stats->rounds = d + 1;
   }
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
@@ -673,6 +682,7 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
     float m = 0.f;
     HIP_TRY(hipEventElapsedTime(&m, kev[k], kev[k + 1]));
     eval_ms += m;
+    stats->launch_seconds[kslot[k / 2] / CPL_NUM_FAMILIES][kslot[k / 2] % CPL_NUM_FAMILIES] = m * 1e-3;
     (void)hipEventDestroy(kev[k]);
     (void)hipEventDestroy(kev[k + 1]);
   }

@@ -43,7 +43,7 @@ struct RefineParams {
   uint64_t n_roots;
   const double* la_table; // per la cell: lb, ub, lbrsp; level d starts at grid[0]*(2^d - 1)
   uint32_t* root_flags;   // bytes, (root * nfam + slot), accessed as words
-  RoundStats* stats;      // [CPL_MAX_ROUNDS]
+  RoundStats* stats;      // [CPL_MAX_ROUNDS][CPL_NUM_FAMILIES]
   Worklist out;           // next-round worklist region of this family
   uint32_t* out_count;
   uint64_t out_cap;

@@ -114,6 +114,8 @@ typedef struct {
   uint64_t verdicts[CPL_MAX_ROUNDS][8];            /* per round, count of each verdict code */
   uint64_t digest[CPL_MAX_ROUNDS];                 /* sum of item hashes per round (order independent) */
   uint64_t verdict_digest[CPL_MAX_ROUNDS];         /* sum of hash(item, verdict byte) per round */
+  uint64_t family_verdicts[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES][8]; /* per round and family */
+  double launch_seconds[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES];        /* device time per launch (0 on the CPU) */
   uint64_t total_evaluations;
   uint64_t certified;                              /* leaves settled PROVEN, EMPTY or OOD */
   uint64_t unresolved;                             /* leaves CRITICAL at max depth, NOFIT or TRAP */

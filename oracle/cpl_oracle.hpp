@@ -34,6 +34,23 @@
 
 namespace cplo {
 
+// Optional FP64 operation counter (liboracle_cpl_count.so is built with
+// -DCPLO_COUNT_FLOPS): every IEEE operation of the restated formulas counts 1,
+// a fused multiply-add 2; comparisons, min/max and ulp steps count 0.
+#ifdef CPLO_COUNT_FLOPS
+#define CPLO_COUNTING 1
+#else
+#define CPLO_COUNTING 0
+#endif
+inline uint64_t& flop_count() {
+  static thread_local uint64_t n = 0;
+  return n;
+}
+#define CPLO_FLOPS(k) \
+  do {                \
+    if (CPLO_COUNTING) cplo::flop_count() += (k); \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // Directed rounding on x86-64 SSE2 (MXCSR bits 13-14: 00 RN, 01 RD, 10 RU).
 // ---------------------------------------------------------------------------
@@ -45,6 +62,7 @@ alignas(4) inline const unsigned ru = 0x5F80u;
 
 #define CPLO_DIRECTED_BINOP(NAME, INSN, MODE)                                  \
   static inline double NAME(double a, double b) {                             \
+    CPLO_FLOPS(1);                                                            \
     asm("ldmxcsr %[m]\n\t" INSN " %[b], %[a]\n\tldmxcsr %[n]"                 \
         : [a] "+x"(a)                                                         \
         : [b] "x"(b), [m] "m"(MODE), [n] "m"(mxcsr::rn));                    \
@@ -61,6 +79,7 @@ CPLO_DIRECTED_BINOP(div_ru, "divsd", mxcsr::ru)
 #undef CPLO_DIRECTED_BINOP
 
 static inline double sqrt_rd(double a) {
+  CPLO_FLOPS(1);
   double r;
   asm("ldmxcsr %[m]\n\tsqrtsd %[a], %[r]\n\tldmxcsr %[n]"
       : [r] "=x"(r)
@@ -68,6 +87,7 @@ static inline double sqrt_rd(double a) {
   return r;
 }
 static inline double sqrt_ru(double a) {
+  CPLO_FLOPS(1);
   double r;
   asm("ldmxcsr %[m]\n\tsqrtsd %[a], %[r]\n\tldmxcsr %[n]"
       : [r] "=x"(r)
@@ -239,6 +259,8 @@ static inline double asin_poly_eval(double z) {
 static inline double asin_rn(double x) {
   double ax = std::fabs(x);
   double r;
+  // 12 fma of the polynomial + 3 (small branch) / 16 (reduced branch)
+  CPLO_FLOPS(ax <= 0.5 ? 24 + 4 : 24 + 18);
   if (ax <= 0.5) {
     double z = ax * ax;
     double p = asin_poly_eval(z);
@@ -246,7 +268,7 @@ static inline double asin_rn(double x) {
   } else {
     double t = 0.5 - 0.5 * ax;           // exact for ax in [0.5, 1]
     double s = std::sqrt(t);
-    double c = std::fma(-s, s, t) / (s + s);  // sqrt(t) ~= s + c
+    double c = t > 0.0 ? std::fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
     double p = asin_poly_eval(t);
     double w = std::fma(s * t, p, c);    // asin(s) ~= s + w
     r = asin_pio2_hi - ((s + s) + (w + w - asin_pio2_lo));
@@ -280,6 +302,7 @@ static inline IV asin(const IV& x) {
 // fused multiply-add by default; the restatement writes the fma explicitly.
 // ---------------------------------------------------------------------------
 static inline IV get_subinterval(IV iv, int i, int n) {
+  CPLO_FLOPS(i + 1 == n ? 4 : 6);
   double lb = iv.lb, ub = iv.ub;
   double diff = ub - lb;
   diff /= n;
@@ -755,6 +778,7 @@ static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3) {
 // r3 from a position interval p in [0,1] over the admissible r3 range; for a
 // root cell p = [k/n, (k+1)/n] this equals get_subinterval(range, k, n).
 static inline IV r3_from_position(IV range, IV p) {
+  CPLO_FLOPS(p.ub == 1.0 ? 3 : 5);
   double w = range.ub - range.lb;
   double l = std::fma(w, p.lb, range.lb);
   double u = (p.ub == 1.0) ? range.ub : std::fma(w, p.ub, range.lb);

@@ -73,6 +73,8 @@ double lbrsp_memo(RunState& st, double la_lb) {
 struct LocalStats {
   uint64_t evaluations[CPL_MAX_ROUNDS] = {};
   uint64_t verdicts[CPL_MAX_ROUNDS][8] = {};
+  uint64_t fam_verdicts[CPL_MAX_ROUNDS][NUM_FAMILIES][8] = {};
+  uint64_t flops[NUM_FAMILIES][8] = {};   // counting build only: flops by family and verdict
   uint64_t digest[CPL_MAX_ROUNDS] = {};
   uint64_t vdigest[CPL_MAX_ROUNDS] = {};
   uint64_t certified = 0, unresolved = 0;
@@ -107,10 +109,18 @@ Item child_of(const Item& p, int axis, int c) {
 void process(RunState& st, LocalStats& ls, const Item& it, uint8_t* flags_out) {
   const cpl_refine_cfg* cfg = st.cfg;
   IV la(it.e[0], it.e[1]), r1(it.e[2], it.e[3]), r2(it.e[4], it.e[5]), p3(it.e[6], it.e[7]);
-  double lbrsp = cfg->clip ? lbrsp_memo(st, la.lb) : 0.0;
+  double lbrsp = 0.0;
+  if (cfg->clip) {
+    uint64_t saved = flop_count();
+    lbrsp = lbrsp_memo(st, la.lb);  // a table lookup on the device: not counted
+    flop_count() = saved;
+  }
+  uint64_t f0 = flop_count();
   uint8_t vb = eval_synthetic(it.family, la, r1, r2, p3, cfg->clip != 0, lbrsp);
   int v = vb & 7;
   int d = it.depth;
+  ls.flops[it.family][v] += flop_count() - f0;
+  ls.fam_verdicts[d][it.family][v]++;
   uint64_t h = item_hash(it);
   ls.evaluations[d]++;
   ls.verdicts[d][v]++;
@@ -178,6 +188,8 @@ int hw_threads(int requested) {
 }  // namespace
 
 extern "C" {
+
+int cplo_counting_build(void) { return CPLO_COUNTING; }
 
 int cplo_prim(int op, const double* a, const double* b, double* out, size_t n) {
   for (size_t i = 0; i < n; ++i) {
@@ -274,8 +286,11 @@ int cplo_eval_cells(int family, int clip, const double* const soa[8], size_t n, 
 
 // Synthetic refinement over roots [root_begin, root_end) (all roots when
 // root_end == 0).  root_flags: (root_end - root_begin) * nfam bytes.
+// flops_out (may be NULL): 6 x 8 sums of counted FP64 operations by family
+// and verdict (nonzero only in the counting build, liboracle_cpl_count.so).
 int cplo_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags,
-                    uint64_t root_begin, uint64_t root_end, uint64_t root_stride, int nthreads) {
+                    uint64_t root_begin, uint64_t root_end, uint64_t root_stride, int nthreads,
+                    uint64_t* flops_out) {
   if (!cfg || !stats) return CPL_ERR_ARG;
   if (cfg->max_depth < 0 || cfg->max_depth >= CPL_MAX_ROUNDS) return CPL_ERR_ARG;
   RunState st;
@@ -316,12 +331,18 @@ int cplo_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t*
     for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
       stats->evaluations[d] += ls.evaluations[d];
       for (int v = 0; v < 8; ++v) stats->verdicts[d][v] += ls.verdicts[d][v];
+      for (int f = 0; f < NUM_FAMILIES; ++f)
+        for (int v = 0; v < 8; ++v) stats->family_verdicts[d][f][v] += ls.fam_verdicts[d][f][v];
       stats->digest[d] += ls.digest[d];
       stats->verdict_digest[d] += ls.vdigest[d];
     }
     stats->certified += ls.certified;
     stats->unresolved += ls.unresolved;
   }
+  if (flops_out)
+    for (const LocalStats& ls : locals)
+      for (int f = 0; f < NUM_FAMILIES; ++f)
+        for (int v = 0; v < 8; ++v) flops_out[f * 8 + v] += ls.flops[f][v];
   int rounds = 0;
   for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
     stats->total_evaluations += stats->evaluations[d];

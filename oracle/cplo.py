@@ -13,7 +13,8 @@ from circlepacking_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_cpl.so")
-_lib = None
+COUNT_LIB_PATH = os.path.join(HERE, "liboracle_cpl_count.so")
+_libs = {}
 _P_D = ctypes.POINTER(ctypes.c_double)
 _P_U8 = ctypes.POINTER(ctypes.c_uint8)
 
@@ -23,24 +24,26 @@ def build():
     return LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(counting=False):
+    """The oracle library; counting=True loads the FP64-operation-counting build."""
+    path = COUNT_LIB_PATH if counting else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.cplo_prim.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.c_size_t]
         L.cplo_geom.argtypes = [ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
         L.cplo_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
         L.cplo_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8,
                                       ctypes.c_int]
         L.cplo_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8,
-                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint64)]
         L.cplo_first_problematic.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(ctypes.c_int32), _P_D, ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_int), ctypes.c_int]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dptr(a):
@@ -87,8 +90,11 @@ def eval_cells(family, soa, clip=True, nthreads=0):
     return out
 
 
-def refine_run(cfg, want_flags=True, root_begin=0, root_end=0, root_stride=1, nthreads=0):
+def refine_run(cfg, want_flags=True, root_begin=0, root_end=0, root_stride=1, nthreads=0, counting=False):
+    """Synthetic refinement on the CPU over roots root_begin, +stride, ... < root_end.
+    Returns (stats, flags) or, with counting=True, (stats, flags, flops[6, 8])."""
     st = abi.RefineStats()
+    flops = np.zeros(48, dtype=np.uint64)
     n_roots = cfg.n_roots()
     end = root_end or n_roots
     count = max(0, (end - root_begin + root_stride - 1) // root_stride)
@@ -97,8 +103,11 @@ def refine_run(cfg, want_flags=True, root_begin=0, root_end=0, root_stride=1, nt
     if want_flags:
         flags = np.zeros((count, len(cfg.families())), dtype=np.uint8)
         fp = flags.ctypes.data_as(_P_U8)
-    rc = lib().cplo_refine_run(ctypes.byref(cfg), ctypes.byref(st), fp, root_begin, root_end, root_stride, nthreads)
+    rc = lib(counting).cplo_refine_run(ctypes.byref(cfg), ctypes.byref(st), fp, root_begin, root_end, root_stride,
+                                       nthreads, flops.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     assert rc == 0
+    if counting:
+        return st, flags, flops.reshape(6, 8)
     return st, flags
 
 

@@ -1,0 +1,73 @@
+"""The C++ lambda-queue driver (run_kernel_until_no_criticals semantics of
+run_kernel.cuh:61-132) driven by a synthetic launcher on the CPU."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def trace_bin(tmp_path_factory):
+    out = tmp_path_factory.mktemp("drv") / "driver_trace"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(out), os.path.join(ROOT, "tests", "driver_trace.cpp")],
+                   check=True)
+    return str(out)
+
+
+def run(trace_bin, interrupt_at=-1):
+    p = subprocess.run([trace_bin, str(interrupt_at)], capture_output=True, text=True, check=True)
+    lines = p.stdout.strip().splitlines()
+    launches = [tuple(float(x) for x in l.split()) for l in lines[:-1]]
+    return launches, int(lines[-1].split()[1]), p.stderr
+
+
+def test_queue_certifies_the_whole_lambda_range(trace_bin):
+    launches, ok, log = run(trace_bin)
+    assert ok == 1
+    assert all(out == 0 for *_, out in launches)
+    # processed in order of lower bound (min-heap), within the outer interval
+    lbs = [l for l, *_ in launches]
+    assert lbs == sorted(lbs) and lbs[0] == 0.5 and max(u for _, u, _, _ in launches) == 0.9900000000000001
+    # certified prefixes [lb, first) of successive launches tile [0.5, 0.99] without gaps
+    covered = []
+    for l, u, f, _ in launches:
+        covered.append((l, u if f == float("inf") else f))
+    covered.sort()
+    assert covered[0][0] == 0.5
+    reach = 0.5
+    for l, u in covered:
+        assert l <= reach
+        reach = max(reach, u)
+    assert reach == 0.9900000000000001
+    # every failing launch re-queues exactly per the split rule
+    assert log.count("Handling lambda interval") == len(launches)
+
+
+def test_split_rule(trace_bin):
+    launches, _, _ = run(trace_bin)
+    queued = {(l, u) for l, u, _, _ in launches[1:]}
+    for l, u, f, _ in launches:
+        if f == float("inf"):
+            continue
+        if (u - l) >= 2 * (u - f):
+            assert (f, u) in queued
+        else:
+            mid = 0.5 * (f + u)
+            assert (f, mid) in queued and (mid, u) in queued
+
+
+def test_sigint_reruns_first_remaining_interval_with_output(trace_bin):
+    full, _, _ = run(trace_bin)
+    k = 3
+    launches, ok, log = run(trace_bin, interrupt_at=k)
+    assert ok == 0
+    assert "We were interrupted" in log
+    # launches 0..k as in the uninterrupted run, then one re-run with output
+    assert launches[:k + 1] == full[:k + 1]
+    last = launches[-1]
+    assert last[3] == 1
+    interrupted = launches[k]
+    if interrupted[2] != float("inf"):
+        assert last[:2] == interrupted[:2]

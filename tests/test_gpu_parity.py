@@ -1,4 +1,6 @@
 """Cell verdicts and refinement runs of the HIP path, identical to the CPU oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -27,6 +29,8 @@ def test_small_grid_cells(cpl, cplo, family, clip):
     got = cpl.eval_cells(family, soa, clip)
     want = cplo.eval_cells(family, soa, clip)
     assert np.array_equal(got, want)
+    golden = np.load(os.path.join(os.path.dirname(__file__), "golden", "small_grid.npz"))
+    assert np.array_equal(got, golden[f"v_{family}_{int(clip)}"])
 
 
 def _compare_runs(g, o, gf, of):

@@ -53,6 +53,14 @@ def test_asin_bit_exact(cpl, cplo, op):
     assert_bit_exact(cpl.prim(op, x), cplo.prim(op, x), x, None, op)
 
 
+def test_asin_endpoints(cpl):
+    x = np.array([1.0, -1.0, 0.0, -0.0, 0.5])
+    y = cpl.prim("asin_rn", x)
+    assert y[0] == 1.5707963267948966 and y[1] == -1.5707963267948966 and y[2] == 0.0 and y[4] > 0.5
+    u = np.linspace(-1, 1, 100001)
+    assert not np.any(np.isnan(cpl.prim("asin_rn", u)))
+
+
 def test_single_placement_bound(cpl, cplo):
     lam = np.concatenate([np.linspace(0.5, 0.9900000000000001, 2049), [0.0, 0.25, 0.999]])
     assert_bit_exact(cpl.prim("lbrsp", lam), cplo.prim("lbrsp", lam), lam, None, "lbrsp")
