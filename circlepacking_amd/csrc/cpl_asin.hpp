@@ -31,7 +31,7 @@ __device__ __forceinline__ double asin_poly_eval(double z) {
   return p;
 }
 
-__device__ __attribute__((noinline)) double asin_rn(double x) {
+__device__ __forceinline__ double asin_rn(double x) {
   double ax = __builtin_fabs(x);
   double r;
   if (ax <= 0.5) {

@@ -85,22 +85,33 @@ __device__ __forceinline__ IV operator*(const IV& a, const IV& b) {
   return IV(ll1 < ll2 ? ll1 : ll2, uu1 > uu2 ? uu1 : uu2);
 }
 
-// Interval / interval with sign cases (interval.cuh:277-316).
+// Interval / interval with the sign cases of interval.cuh:277-316.  The six
+// cases differ only in which endpoints are divided, so the operands are
+// selected branch-free and one directed pair is computed; a divisor that
+// contains zero gives (-inf, +inf) as in the reference.
 __device__ __forceinline__ IV operator/(const IV& a, const IV& o) {
-  double l, u;
-  if (o.lb > 0) {
-    if (a.lb >= 0) { l = div_rd(a.lb, o.ub); u = div_ru(a.ub, o.lb); }
-    else if (a.ub <= 0) { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.ub); }
-    else { l = div_rd(a.lb, o.lb); u = div_ru(a.ub, o.lb); }
-  } else if (o.ub < 0) {
-    if (a.lb >= 0) { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.lb); }
-    else if (a.ub <= 0) { l = div_rd(a.ub, o.lb); u = div_ru(a.lb, o.ub); }
-    else { l = div_rd(a.ub, o.ub); u = div_ru(a.lb, o.ub); }
+  const bool pos = o.lb > 0, neg = o.ub < 0;
+  const bool anneg = a.lb >= 0, anpos = a.ub <= 0;  // a >= 0, a <= 0
+  double nl, dl, nu, du;
+  if (pos) {
+    nl = a.lb;
+    dl = anneg ? o.ub : o.lb;
+    nu = a.ub;
+    du = (anneg || !anpos) ? o.lb : o.ub;
   } else {
-    u = __builtin_huge_val();
-    l = -u;
+    nl = a.ub;
+    dl = (anneg || !anpos) ? o.ub : o.lb;
+    nu = a.lb;
+    du = anneg ? o.lb : o.ub;
   }
-  return IV(l, u);
+  if (!pos && !neg) { nl = nu = 0.0; dl = du = 1.0; }
+  IV r;
+  div_pair(nl, dl, nu, du, r.lb, r.ub);
+  if (!pos && !neg) {
+    r.ub = __builtin_huge_val();
+    r.lb = -r.ub;
+  }
+  return r;
 }
 
 __device__ __forceinline__ UB operator<(const IV& a, const IV& b) { return UB{a.ub < b.lb, a.lb < b.ub}; }

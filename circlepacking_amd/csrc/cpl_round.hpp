@@ -25,7 +25,7 @@
 //     r = fma(-q, b, a)          (the exact remainder, rounded toward the target)
 //     result = fma(r, y, q)      (y a power of two ~ 1/(2|b|))
 // whose directed rounding lands on q or its neighbour exactly as the rounded
-// true quotient does (Markstein-style correction; see div_dir below).
+// true quotient does (Markstein-style correction; see div_fast_ok below).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -127,20 +127,6 @@ CPL_SCALE_DIR(scale_rd, CPL_MODE_RD)
 CPL_SCALE_DIR(scale_ru, CPL_MODE_RU)
 #undef CPL_SCALE_DIR
 
-// Directed quotient with b > 0 normalised, both finite and nonzero, and every
-// exponent inside the "comfortable" range so the remainder is exact:
-// |a|, |b|, |q| in [2^-960, 2^960].
-template <bool UP>
-__device__ __forceinline__ double div_core(double a, double b, double q) {
-  double y = pow2(1022 - dexp_field(b));  // 2^-(e_b+1): e_b = field - 1023
-  return UP ? div_fix_ru(q, b, a, y) : div_fix_rd(q, b, a, y);
-}
-
-__device__ __forceinline__ bool exp_comfortable(double d) {
-  unsigned e = (unsigned)dexp_field(d);
-  return e - 63u < 1921u;  // field in [63, 1983]  <=>  |d| in [2^-960, 2^961)
-}
-
 // Slow path: zeros, infinities, NaN, overflow, subnormal operands or results.
 // Both operands are scaled by exact powers of two into [1, 2), the scaled
 // quotient is rounded in the target mode (exact remainder there), and the
@@ -166,19 +152,41 @@ __device__ __attribute__((noinline)) double div_slow(double a, double b) {
   return UP ? scale_ru(ds, pow2(shift)) : scale_rd(ds, pow2(shift));
 }
 
+// Fast-path condition: |q| and |b| in [2^-480, 2^480).  Then |a| = |q b|(1+e)
+// lies in [2^-961, 2^961], so the remainder a - q*b is exactly representable
+// and y = +-2^-600 satisfies 0 < y*b <= 2^-120 < 1: q + r*y is strictly between
+// q and the true quotient (the fma computes r*y exactly, however small).
+__device__ __forceinline__ bool div_fast_ok(double q, double b) {
+  double aq = __builtin_fabs(q), ab = __builtin_fabs(b);
+  return (aq >= 0x1p-480) & (aq < 0x1p+480) & (ab >= 0x1p-480) & (ab < 0x1p+480);
+}
+__device__ __forceinline__ double div_y(double b) { return __builtin_copysign(0x1p-600, b); }
+
 template <bool UP>
 __device__ __forceinline__ double div_dir(double a, double b) {
   double q = a / b;  // IEEE round-to-nearest
-  if (__builtin_expect(exp_comfortable(a) & exp_comfortable(b) & exp_comfortable(q), 1)) {
-    // a/b = (-a)/(-b): normalise the divisor positive, the quotient is unchanged
-    double sa = b < 0.0 ? -a : a;
-    double sb = b < 0.0 ? -b : b;
-    return div_core<UP>(sa, sb, q);
-  }
+  if (__builtin_expect(div_fast_ok(q, b), 1)) return UP ? div_fix_ru(q, b, a, div_y(b)) : div_fix_rd(q, b, a, div_y(b));
+  if (a == 0.0) return q;  // 0/b: the exact signed zero (or NaN for 0/0)
   return div_slow<UP>(a, b);
 }
 __device__ __forceinline__ double div_rd(double a, double b) { return div_dir<false>(a, b); }
 __device__ __forceinline__ double div_ru(double a, double b) { return div_dir<true>(a, b); }
+
+// [rd(al / bl), ru(au / bu)]: both corrections in one statement.
+__device__ __forceinline__ void div_pair(double al, double bl, double au, double bu, double& l, double& u) {
+  double ql = al / bl, qu = au / bu;
+  if (__builtin_expect(div_fast_ok(ql, bl) & div_fast_ok(qu, bu), 1)) {
+    double yl = div_y(bl), yu = div_y(bu), rl, ru;
+    asm("s_nop 1\n\t" CPL_MODE_RD "v_fma_f64 %2, -%4, %5, %6\n\t"
+        "v_fma_f64 %0, %2, %7, %4\n\t" CPL_MODE_RU "v_fma_f64 %3, -%8, %9, %10\n\t"
+        "v_fma_f64 %1, %3, %11, %8\n\t" CPL_MODE_RN
+        : "=&v"(l), "=&v"(u), "=&v"(rl), "=&v"(ru)
+        : "v"(ql), "v"(bl), "v"(al), "v"(yl), "v"(qu), "v"(bu), "v"(au), "v"(yu));
+    return;
+  }
+  l = div_fast_ok(ql, bl) ? div_fix_rd(ql, bl, al, div_y(bl)) : (al == 0.0 ? ql : div_slow<false>(al, bl));
+  u = div_fast_ok(qu, bu) ? div_fix_ru(qu, bu, au, div_y(bu)) : (au == 0.0 ? qu : div_slow<true>(au, bu));
+}
 
 // ---- square root ------------------------------------------------------------
 // s = RN(sqrt x); r = fma(-s, s, x) and fma(r, y, s) in the target mode with
