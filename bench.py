@@ -43,7 +43,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="plain", choices=["plain"])
+    ap.add_argument("--variant", default="tuned", choices=["plain", "tuned"],
+                    help="which form `value` reports; both are timed and checked for identical results")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use 256 threads, 1 cell/thread)")
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--depth", type=int, default=6)
     ap.add_argument("--cpu-stride", type=int, default=5, help="CPU baseline runs every S-th root cell")
@@ -57,17 +59,34 @@ def main():
     import circlepacking_amd as cpl
     from circlepacking_amd import abi
 
-    cfg = abi.cfg_4param(n=args.grid, max_depth=args.depth)
-    for _ in range(args.warmup):
-        cpl.refine_run(cfg, want_flags=False)
-    runs = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st, _ = cpl.refine_run(cfg, want_flags=False)  # synchronous: returns after the device finished
-        runs.append(st)
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    st = runs[-1]
+    base = abi.cfg_4param(n=args.grid, max_depth=args.depth)
+    plain_cfg = abi.with_options(base, abi.OPT_NO_DIGEST)
+    tuned_cfg = abi.with_options(base, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST)
+    tune_table = None
+    if not args.no_tune:
+        tune_table = cpl.autotune(tuned_cfg, reps=3)
+
+    def timed(cfg):
+        for _ in range(args.warmup):
+            cpl.refine_run(cfg, want_flags=False)
+        runs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st, _ = cpl.refine_run(cfg, want_flags=False)  # synchronous: returns after the device finished
+            runs.append(st)
+        t1 = time.perf_counter()
+        return (t1 - t0) * 1e3 / args.steps, runs[-1]
+
+    ms_plain, st_plain = timed(plain_cfg)
+    ms_tuned, st_tuned = timed(tuned_cfg)
+    # identical verdicts: full digests and per-root flags of both forms (untimed)
+    vp, fp = cpl.refine_run(abi.with_options(base, 0))
+    vt, ft = cpl.refine_run(abi.with_options(tuned_cfg, tuned_cfg.options & ~abi.OPT_NO_DIGEST))
+    identical = vp.as_dict() == vt.as_dict() and bool(np.array_equal(fp, ft))
+    if args.variant == "plain":
+        ms_per_step, st = ms_plain, st_plain
+    else:
+        ms_per_step, st = ms_tuned, st_tuned
     certified = st.certified
     value = certified / (ms_per_step * 1e-3)
 
@@ -79,7 +98,7 @@ def main():
 
         threads = min(16, os.cpu_count() or 1)
         c0 = time.perf_counter()
-        ost, _ = cplo.refine_run(cfg, want_flags=False, root_stride=args.cpu_stride, nthreads=threads)
+        ost, _ = cplo.refine_run(base, want_flags=False, root_stride=args.cpu_stride, nthreads=threads)
         c1 = time.perf_counter()
         cpu = {
             "value": ost.certified / (c1 - c0),
@@ -89,7 +108,7 @@ def main():
             "sample": f"every {args.cpu_stride}th root cell of the same 64^4 grid ({ost.total_evaluations} "
                       f"evaluations, {ost.certified} certified leaves, {c1 - c0:.1f} s wall)",
         }
-        fst, _, flops = cplo.refine_run(cfg, want_flags=False, root_stride=max(args.cpu_stride, 97),
+        fst, _, flops = cplo.refine_run(base, want_flags=False, root_stride=max(args.cpu_stride, 97),
                                         nthreads=threads, counting=True)
         counts = np.zeros((6, 8))
         for d in range(fst.rounds):
@@ -98,7 +117,8 @@ def main():
             flops_per_eval = np.where(counts > 0, flops / np.maximum(counts, 1), 0.0)
 
     # ---- roofline of the dominant kernel (largest total device time) ----
-    groups = kernel_groups(st, cfg)
+    cfg = plain_cfg
+    groups = kernel_groups(st_plain, cfg)
     roof = None
     if groups:
         key = max(groups, key=lambda k: sum(s for _, s, _ in groups[k]))
@@ -146,7 +166,15 @@ def main():
             "grid": [args.grid] * 4,
             "max_depth": args.depth,
             "variant": args.variant,
+            "tuned_options": "hipGraph rounds + hierarchical roots" if args.variant == "tuned" else None,
+            "block": tuned_cfg.block if args.variant == "tuned" else 256,
+            "cells_per_thread": tuned_cfg.cells_per_thread if args.variant == "tuned" else 1,
         },
+        "plain_ms_per_step": ms_plain,
+        "tuned_ms_per_step": ms_tuned,
+        "tuned_speedup": ms_plain / ms_tuned,
+        "identical_verdicts": identical,
+        "autotune": tune_table,
         "time_to_proof_ms": ms_per_step,
         "device_ms_per_step": st.seconds * 1e3,
         "evaluations": st.total_evaluations,

@@ -37,6 +37,8 @@ def lib():
         L.cpl_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8]
         L.cpl_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
         L.cpl_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8]
+        L.cpl_autotune.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.c_int, ctypes.POINTER(abi.TunePoint),
+                                   ctypes.POINTER(ctypes.c_int)]
         L.cpl_first_problematic.argtypes = [ctypes.c_int, abi.Iv, ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
                                             _P_D, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
@@ -106,3 +108,11 @@ def first_problematic(family, outer_la, subdiv=None, output_criticals=False, tex
                                      ctypes.byref(ev), buf, text_cap if buf is not None else 0)
     _check(rc)
     return first.value, ev.value, (buf.value.decode() if buf is not None else "")
+
+
+def autotune(cfg, reps=3):
+    """Pick block size and cells-per-thread for cfg (in place); returns the timing table."""
+    pts = (abi.TunePoint * 20)()
+    n = ctypes.c_int()
+    _check(lib().cpl_autotune(ctypes.byref(cfg), reps, pts, ctypes.byref(n)))
+    return [(p.block, p.cells_per_thread, p.seconds) for p in pts[: n.value]]

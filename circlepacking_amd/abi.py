@@ -14,6 +14,8 @@ CPL_OK = 0
 ERR_NAMES = {-1: "HIP", -2: "ARG", -3: "TRAP", -4: "NOFIT", -5: "CAPACITY", -6: "NO_DEVICE"}
 MAX_ROUNDS = 16
 
+OPT_GRAPH, OPT_PERSISTENT, OPT_HIER_ROOTS, OPT_FP32, OPT_LEARNED, OPT_NO_DIGEST = 1, 2, 4, 8, 16, 256
+
 LAMBDA_MIN = 0.5
 LAMBDA_MAX = 0.9900000000000001
 
@@ -48,7 +50,10 @@ class RefineCfg(ctypes.Structure):
         ("max_depth", ctypes.c_int32),
         ("split_mode", ctypes.c_int32),
         ("split_axes", ctypes.c_uint32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("options", ctypes.c_uint32),
+        ("block", ctypes.c_int32),
+        ("cells_per_thread", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
     def families(self):
@@ -59,6 +64,10 @@ class RefineCfg(ctypes.Structure):
         for g in self.grid:
             n *= g
         return n
+
+
+class TunePoint(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32), ("seconds", ctypes.c_double)]
 
 
 class RefineStats(ctypes.Structure):
@@ -94,7 +103,7 @@ class RefineStats(ctypes.Structure):
         }
 
 
-def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_axes):
+def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_axes, options=0, block=0, cpt=0):
     c = RefineCfg()
     c.grid[:] = list(grid)
     c.box_lb[:] = list(box_lb)
@@ -104,6 +113,20 @@ def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_
     c.max_depth = int(max_depth)
     c.split_mode = int(split_mode)
     c.split_axes = int(split_axes)
+    c.options = int(options)
+    c.block = int(block)
+    c.cells_per_thread = int(cpt)
+    return c
+
+
+def with_options(cfg, options, block=None, cpt=None):
+    """Copy of cfg with other execution options (and optionally shape)."""
+    c = RefineCfg.from_buffer_copy(cfg)
+    c.options = int(options)
+    if block is not None:
+        c.block = int(block)
+    if cpt is not None:
+        c.cells_per_thread = int(cpt)
     return c
 
 

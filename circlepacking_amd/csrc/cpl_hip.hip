@@ -86,8 +86,10 @@ __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, 
 
 // One round of family F.  ROOT: items are the implicit root grid (n = n_roots),
 // else items come from `in` (n items).
-template <int F, bool ROOT>
-__global__ __launch_bounds__(256) void k_round(RefineParams P, Worklist in, uint64_t n) {
+// LB: launch bound (maximum threads per block) the register allocation is made for.
+template <int F, bool ROOT, int LB = 256>
+__global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint64_t n) {
+  if (P.in_count) n = *P.in_count;
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
@@ -128,11 +130,13 @@ __global__ __launch_bounds__(256) void k_round(RefineParams P, Worklist in, uint
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
       vb = eval_synthetic<F>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp);
       int v = vb & 7;
-      uint64_t h = item_hash(e, root, F, meta);
       c_eval += 1;
       c_v[v] += 1;
-      c_dig += h;
-      c_vdig += mix64(h ^ vb);
+      if (P.digest) {
+        uint64_t h = item_hash(e, root, F, meta);
+        c_dig += h;
+        c_vdig += mix64(h ^ vb);
+      }
       uint64_t fbyte = (uint64_t)root * P.nfam + P.fam_slot;
       uint32_t fl = (vb & 8) ? CPL_RF_NOFIT_FLAG : 0;
       if (v == V_CRITICAL && depth < P.max_depth) {
@@ -230,6 +234,105 @@ __global__ __launch_bounds__(256) void k_round(RefineParams P, Worklist in, uint
   wave_add(&S->vdigest, c_vdig);
   wave_add(&S->certified, c_cert);
   wave_add(&S->unresolved, c_unres);
+}
+
+// Root round with admissibility decided once per (la, r1, r2) triple: an
+// out-of-domain triple accounts for its grid[3] root items (counts, digests,
+// flags) without evaluating them; the items of a live triple are appended to
+// `P.out` (depth 0) for k_round to evaluate.  The resulting counts, digests and
+// flags equal those of evaluating every root item.
+template <int F>
+__global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t n_tri = (uint64_t)P.grid[0] * P.grid[1] * P.grid[2];
+  const int n3 = P.grid[3];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long c_ood = 0, c_dig = 0, c_vdig = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_tri; base += stride) {
+    uint64_t t = base + threadIdx.x;
+    bool live = false;
+    double e[8];
+    int i0 = 0;
+    if (t < n_tri) {
+      uint64_t r = t;
+      int i2 = (int)(r % (uint64_t)P.grid[2]); r /= (uint64_t)P.grid[2];
+      int i1 = (int)(r % (uint64_t)P.grid[1]); r /= (uint64_t)P.grid[1];
+      i0 = (int)r;
+      e[0] = P.la_table[3 * i0 + 0];
+      e[1] = P.la_table[3 * i0 + 1];
+      IV r1 = get_subinterval(IV(P.box_lb[1], P.box_ub[1]), i1, P.grid[1]);
+      IV r2 = get_subinterval(IV(P.box_lb[2], P.box_ub[2]), i2, P.grid[2]);
+      e[2] = r1.lb; e[3] = r1.ub; e[4] = r2.lb; e[5] = r2.ub;
+      IV la(e[0], e[1]);
+      bool ood = false;
+      IV r1c = r1, r2c = r2;
+      if (P.clip) {
+        r1c = r1.intersect(compute_r1_range(la, P.la_table[3 * i0 + 2]));
+        ood = r1c.empty();
+        if (!ood) {
+          r2c = r2.intersect(r2_range<F>(la, r1c));
+          ood = r2c.empty();
+        }
+      }
+      if constexpr (is_3d(F)) {
+        if (!ood) ood = r3_range<F>(la, r2c).empty();
+      }
+      live = !ood;
+      if (ood) {
+        c_ood += (unsigned long long)n3;
+        for (int k = 0; k < n3; ++k) {
+          uint32_t root = (uint32_t)(t * n3 + k);
+          if (P.digest) {
+            IV p = get_subinterval(IV(P.box_lb[3], P.box_ub[3]), k, n3);
+            e[6] = p.lb; e[7] = p.ub;
+            uint64_t h = item_hash(e, root, F, 0u);
+            c_dig += h;
+            c_vdig += mix64(h ^ (uint64_t)vbyte(V_OOD, 0, 0));
+          }
+          if (P.root_flags) flag_root(P.root_flags, (uint64_t)root * P.nfam + P.fam_slot, CPL_RF_OOD);
+        }
+      }
+    }
+    uint64_t mask = __ballot(live);
+    if (mask) {
+      uint32_t cnt = (uint32_t)__popcll(mask);
+      uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      int leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t basepos = 0;
+      if (lane == leader) basepos = atomicAdd(P.out_count, cnt * (uint32_t)n3);
+      basepos = __shfl(basepos, leader, 64);
+      if (live) {
+        uint64_t pos = (uint64_t)basepos + (uint64_t)rank * n3;
+        if (pos + n3 > P.out_cap) {
+          atomicOr(P.overflow, 1u);
+        } else {
+          for (int k = 0; k < n3; ++k) {
+            IV p = get_subinterval(IV(P.box_lb[3], P.box_ub[3]), k, n3);
+            uint64_t o = pos + k;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) P.out.e[j][o] = e[j];
+            P.out.e[6][o] = p.lb;
+            P.out.e[7][o] = p.ub;
+            P.out.root[o] = (uint32_t)(t * n3 + k);
+            P.out.la_idx[o] = (uint32_t)i0;
+            P.out.meta[o] = 0u;
+          }
+        }
+      }
+    }
+  }
+  RoundStats* S = P.stats + 0 * CPL_NUM_FAMILIES + F;
+  wave_add(&S->evaluations, c_ood);
+  wave_add(&S->verdicts[V_OOD], c_ood);
+  wave_add(&S->certified, c_ood);
+  wave_add(&S->digest, c_dig);
+  wave_add(&S->vdigest, c_vdig);
+}
+
+// Fill n words with a value (graph-safe replacement of hipMemsetAsync).
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 template <int F>
@@ -448,6 +551,17 @@ template <int F> struct LaunchRound {
     else hipLaunchKernelGGL((k_round<F, false>), dim3(grid), dim3(256), 0, g.stream, P, in, n);
   }
 };
+template <int F> struct LaunchRoundB {
+  static void run(bool root, unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
+    if (block <= 256) {
+      if (root) hipLaunchKernelGGL((k_round<F, true, 256>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+      else hipLaunchKernelGGL((k_round<F, false, 256>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+    } else {
+      if (root) hipLaunchKernelGGL((k_round<F, true, 1024>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+      else hipLaunchKernelGGL((k_round<F, false, 1024>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+    }
+  }
+};
 template <int F> struct LaunchModeR {
   static void run(unsigned grid, const ModeRParams& P) {
     hipLaunchKernelGGL(k_mode_r<F>, dim3(grid), dim3(256), 0, g.stream, P);
@@ -538,57 +652,76 @@ int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* ver
   return CPL_OK;
 }
 
-int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags) {
-  int rc = ensure_init();
-  if (rc) return rc;
-  if (!cfg || !stats) return fail(CPL_ERR_ARG, "cpl_refine_run: null cfg or stats");
-  if (cfg->max_depth < 0 || cfg->max_depth >= CPL_MAX_ROUNDS) return fail(CPL_ERR_ARG, "max_depth out of range");
+}  // extern "C"
+
+namespace {
+
+// Everything a refinement run derives from its configuration.
+struct Plan {
+  std::vector<int> fams;
+  int nf = 0;
+  int n_children = 2;
+  int la_levels = 1;
   uint64_t n_roots = 1;
+  uint64_t n_triples = 1;
+  uint64_t la_entries = 0;
+  uint64_t cap = 0;
+  bool digest = true;
+  bool flags = false;
+  int block = 256;
+  int cpt = 1;
+  size_t stats_bytes = 0;
+  uint64_t flag_words = 0;
+};
+
+int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
+  if (cfg->max_depth < 0 || cfg->max_depth >= CPL_MAX_ROUNDS) return fail(CPL_ERR_ARG, "max_depth out of range");
+  pl.n_roots = 1;
   for (int a = 0; a < 4; ++a) {
     if (cfg->grid[a] < 1) return fail(CPL_ERR_ARG, "grid counts must be >= 1");
-    n_roots *= (uint64_t)cfg->grid[a];
+    pl.n_roots *= (uint64_t)cfg->grid[a];
   }
-  if (n_roots > 0xFFFFFFFFull) return fail(CPL_ERR_ARG, "more than 2^32 root cells");
+  if (pl.n_roots > 0xFFFFFFFFull) return fail(CPL_ERR_ARG, "more than 2^32 root cells");
+  pl.n_triples = pl.n_roots / (uint64_t)cfg->grid[3];
   if (cfg->split_axes == 0 || (cfg->split_axes & ~0xFu)) return fail(CPL_ERR_ARG, "split_axes must be a nonzero subset of 0xF");
-  std::vector<int> fams;
   for (int f = 0; f < CPL_NUM_FAMILIES; ++f)
-    if (cfg->family_mask & (1u << f)) fams.push_back(f);
-  if (fams.empty()) return fail(CPL_ERR_ARG, "empty family mask");
-  const int nf = (int)fams.size();
-  const int n_children = cfg->split_mode == 0 ? 2 : (1 << __builtin_popcount(cfg->split_axes));
-  const int la_levels = 1 + ((cfg->split_axes & 1) ? cfg->max_depth : 0);
-
-  // ---- device buffers ----
-  const uint64_t la_entries = (uint64_t)cfg->grid[0] * ((1ull << la_levels) - 1);
-  if ((rc = g.la_table.ensure(la_entries * 3 * sizeof(double)))) return rc;
-  const uint64_t flag_bytes = ((n_roots * nf + 3) / 4) * 4;
-  if ((rc = g.root_flags.ensure(flag_bytes))) return rc;
-  const size_t stats_bytes = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats);
-  if ((rc = g.stats.ensure(stats_bytes))) return rc;
-  if ((rc = g.counters.ensure(2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
+    if (cfg->family_mask & (1u << f)) pl.fams.push_back(f);
+  if (pl.fams.empty()) return fail(CPL_ERR_ARG, "empty family mask");
+  pl.nf = (int)pl.fams.size();
+  pl.n_children = cfg->split_mode == 0 ? 2 : (1 << __builtin_popcount(cfg->split_axes));
+  pl.la_levels = 1 + ((cfg->split_axes & 1) ? cfg->max_depth : 0);
+  pl.la_entries = (uint64_t)cfg->grid[0] * ((1ull << pl.la_levels) - 1);
   const char* cap_env = getenv("CPL_WORKLIST_CAP");
-  uint64_t cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
+  pl.cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
+  pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
+  pl.flags = want_flags;
+  pl.block = cfg->block > 0 ? cfg->block : 256;
+  if (pl.block % 64 || pl.block > 1024) return fail(CPL_ERR_ARG, "block must be a multiple of 64 up to 1024");
+  pl.cpt = cfg->cells_per_thread > 0 ? cfg->cells_per_thread : 1;
+  pl.stats_bytes = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats);
+  pl.flag_words = (pl.n_roots * pl.nf + 3) / 4;
+  return CPL_OK;
+}
+
+// Device buffers: worklists [0], [1] (ping-pong) and [2] (live root items),
+// each with one region per family; counters[3][families] + overflow word.
+int ensure_buffers(const Plan& pl) {
+  int rc;
   const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
+  if ((rc = g.la_table.ensure(pl.la_entries * 3 * sizeof(double)))) return rc;
+  if ((rc = g.root_flags.ensure(pl.flag_words * 4 + 4))) return rc;
+  if ((rc = g.stats.ensure(pl.stats_bytes))) return rc;
+  if ((rc = g.counters.ensure(4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
   for (int b = 0; b < 2; ++b)
-    if ((rc = g.wl[b].ensure((size_t)cap * per_item * nf))) return rc;
-  g.wl_cap_items = cap;
+    if ((rc = g.wl[b].ensure((size_t)pl.cap * per_item * pl.nf))) return rc;
+  g.wl_cap_items = pl.cap;
+  return CPL_OK;
+}
 
-  hipEvent_t ev0, ev1;
-  HIP_TRY(hipEventCreate(&ev0));
-  HIP_TRY(hipEventCreate(&ev1));
-  std::vector<hipEvent_t> kev;
-  std::vector<int> kslot;
+uint32_t* counter(int buf, int slot) { return g.counters.as<uint32_t>() + buf * CPL_NUM_FAMILIES + slot; }
+uint32_t* overflow_word() { return g.counters.as<uint32_t>() + 3 * CPL_NUM_FAMILIES; }
 
-  HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, flag_bytes, g.stream));
-  HIP_TRY(hipMemsetAsync(g.stats.p, 0, stats_bytes, g.stream));
-  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 2 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
-  HIP_TRY(hipEventRecord(ev0, g.stream));
-  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
-                     cfg->box_ub[0], cfg->grid[0], la_levels, cfg->clip, g.la_table.as<double>());
-  HIP_TRY(hipGetLastError());
-
-  uint32_t* counters = g.counters.as<uint32_t>();  // [2][nf] next-round counts, then overflow flag
-  uint32_t* overflow = counters + 2 * CPL_NUM_FAMILIES;
+RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   RefineParams P;
   std::memset(&P, 0, sizeof(P));
   for (int a = 0; a < 4; ++a) {
@@ -600,64 +733,39 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   P.max_depth = cfg->max_depth;
   P.split_mode = cfg->split_mode;
   P.split_axes = cfg->split_axes;
-  P.n_children = n_children;
-  P.nfam = nf;
-  P.n_roots = n_roots;
+  P.n_children = pl.n_children;
+  P.nfam = pl.nf;
+  P.n_roots = pl.n_roots;
   P.la_table = g.la_table.as<double>();
-  P.root_flags = g.root_flags.as<uint32_t>();
+  P.root_flags = pl.flags ? g.root_flags.as<uint32_t>() : nullptr;
   P.stats = g.stats.as<RoundStats>();
-  P.overflow = overflow;
-  P.out_cap = cap;
+  P.overflow = overflow_word();
+  P.out_cap = pl.cap;
+  P.digest = pl.digest ? 1 : 0;
+  return P;
+}
 
-  std::vector<uint32_t> cur_n(nf, 0);
-  uint64_t peak = 0;
-  int cur = 0;
-  for (int d = 0; d <= cfg->max_depth; ++d) {
-    const int nxt = cur ^ 1;
-    P.depth = d;
-    bool any = false;
-    for (int s = 0; s < nf; ++s) {
-      uint64_t n = d == 0 ? n_roots : cur_n[s];
-      if (n == 0) continue;
-      any = true;
-      P.fam_slot = s;
-      P.out = region(g.wl[nxt], cap, s);
-      P.out_count = counters + nxt * CPL_NUM_FAMILIES + s;
-      Worklist in = region(g.wl[cur], cap, s);
-      hipEvent_t a, b;
-      HIP_TRY(hipEventCreate(&a));
-      HIP_TRY(hipEventCreate(&b));
-      HIP_TRY(hipEventRecord(a, g.stream));
-      dispatch_family<LaunchRound>(fams[s], d == 0, grid_for(n, 256), P, in, n);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventRecord(b, g.stream));
-      kev.push_back(a);
-      kev.push_back(b);
-      kslot.push_back(d * CPL_NUM_FAMILIES + fams[s]);
-    }
-    if (!any) break;
-    // next-round sizes (one small copy per round) and reset of the consumed counters
-    uint32_t h[CPL_NUM_FAMILIES + 1];
-    HIP_TRY(hipMemcpyAsync(h, counters + nxt * CPL_NUM_FAMILIES, CPL_NUM_FAMILIES * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&h[CPL_NUM_FAMILIES], overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemsetAsync(counters + cur * CPL_NUM_FAMILIES, 0, CPL_NUM_FAMILIES * sizeof(uint32_t), g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    if (h[CPL_NUM_FAMILIES]) return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
-    uint64_t tot = 0;
-    for (int s = 0; s < nf; ++s) {
-      cur_n[s] = h[s];
-      tot += h[s];
-    }
-    peak = std::max(peak, tot);
-    cur = nxt;
+void fill(void* p, uint64_t words, uint32_t v) {
+  unsigned grid = (unsigned)std::min<uint64_t>((words + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::max(grid, 1u)), dim3(256), 0, g.stream, static_cast<uint32_t*>(p), words, v);
+}
+
+template <int F> struct LaunchTriples {
+  static void run(unsigned grid, int block, const RefineParams& P) {
+    hipLaunchKernelGGL(k_root_triples<F>, dim3(grid), dim3(block), 0, g.stream, P);
   }
-  HIP_TRY(hipEventRecord(ev1, g.stream));
-  HIP_TRY(hipStreamSynchronize(g.stream));
+};
 
+int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   std::vector<RoundStats> rs(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES);
-  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, stats_bytes, hipMemcpyDeviceToHost));
-  if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, n_roots * nf, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, pl.stats_bytes, hipMemcpyDeviceToHost));
+  if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, pl.n_roots * pl.nf, hipMemcpyDeviceToHost));
+  double seconds = stats->seconds, eval_seconds = stats->eval_seconds;
+  uint64_t peak = stats->peak_items;
+  double launch[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES];
+  std::memcpy(launch, stats->launch_seconds, sizeof(launch));
   std::memset(stats, 0, sizeof(*stats));
+  std::memcpy(stats->launch_seconds, launch, sizeof(launch));
   for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
     for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
       const RoundStats& r = rs[d * CPL_NUM_FAMILIES + f];
@@ -674,6 +782,73 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
     }
     if (stats->evaluations[d]) stats->rounds = d + 1;
   }
+  stats->seconds = seconds;
+  stats->eval_seconds = eval_seconds;
+  stats->peak_items = peak;
+  return CPL_OK;
+}
+
+// The plain path: one launch per family and round, the host reads the next
+// round's sizes after each round (sizes the grids, detects overflow), HIP
+// events time every launch.
+int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  std::vector<hipEvent_t> kev;
+  std::vector<int> kslot;
+  if (pl.flags) HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, pl.flag_words * 4, g.stream));
+  HIP_TRY(hipMemsetAsync(g.stats.p, 0, pl.stats_bytes, g.stream));
+  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
+  HIP_TRY(hipEventRecord(ev0, g.stream));
+  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
+                     cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
+  HIP_TRY(hipGetLastError());
+  RefineParams P = base_params(cfg, pl);
+  std::vector<uint32_t> cur_n(pl.nf, 0);
+  uint64_t peak = 0;
+  int cur = 0;
+  for (int d = 0; d <= cfg->max_depth; ++d) {
+    const int nxt = cur ^ 1;
+    P.depth = d;
+    bool any = false;
+    for (int s = 0; s < pl.nf; ++s) {
+      uint64_t n = d == 0 ? pl.n_roots : cur_n[s];
+      if (n == 0) continue;
+      any = true;
+      P.fam_slot = s;
+      P.out = region(g.wl[nxt], pl.cap, s);
+      P.out_count = counter(nxt, s);
+      Worklist in = region(g.wl[cur], pl.cap, s);
+      hipEvent_t a, b;
+      HIP_TRY(hipEventCreate(&a));
+      HIP_TRY(hipEventCreate(&b));
+      HIP_TRY(hipEventRecord(a, g.stream));
+      dispatch_family<LaunchRound>(pl.fams[s], d == 0, grid_for(n, 256), P, in, n);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(b, g.stream));
+      kev.push_back(a);
+      kev.push_back(b);
+      kslot.push_back(d * CPL_NUM_FAMILIES + pl.fams[s]);
+    }
+    if (!any) break;
+    uint32_t h[CPL_NUM_FAMILIES + 1];
+    HIP_TRY(hipMemcpyAsync(h, counter(nxt, 0), CPL_NUM_FAMILIES * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&h[CPL_NUM_FAMILIES], overflow_word(), sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemsetAsync(counter(cur, 0), 0, CPL_NUM_FAMILIES * sizeof(uint32_t), g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (h[CPL_NUM_FAMILIES]) return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
+    uint64_t tot = 0;
+    for (int s = 0; s < pl.nf; ++s) {
+      cur_n[s] = h[s];
+      tot += h[s];
+    }
+    peak = std::max(peak, tot);
+    cur = nxt;
+  }
+  HIP_TRY(hipEventRecord(ev1, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  std::memset(stats, 0, sizeof(*stats));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
   stats->seconds = ms * 1e-3;
@@ -690,6 +865,188 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   stats->peak_items = peak;
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  return collect(pl, stats, root_flags);
+}
+
+// ---- graph path ---------------------------------------------------------------
+struct GraphCache {
+  bool valid = false;
+  cpl_refine_cfg cfg;
+  bool flags = false;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  // expected items per (round, slot) from the previous replay: sizes the grids
+  std::vector<uint32_t> expect;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+};
+GraphCache g_graph;
+
+unsigned grid_tuned(uint64_t expect, const Plan& pl) {
+  uint64_t per_block = (uint64_t)pl.block * pl.cpt;
+  uint64_t want = (std::max<uint64_t>(expect, 1) + per_block - 1) / per_block;
+  uint64_t cap = (uint64_t)std::max(g.cu_count, 1) * 64;
+  return (unsigned)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+// Enqueue the whole run (all families, all rounds) with device-side sizes.
+void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<uint32_t>& expect) {
+  const bool hier = (cfg->options & CPL_OPT_HIER_ROOTS) != 0;
+  fill(g.stats.p, pl.stats_bytes / 4, 0u);
+  fill(g.counters.p, 3 * CPL_NUM_FAMILIES + 1, 0u);
+  if (pl.flags) fill(g.root_flags.p, pl.flag_words, 0u);
+  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
+                     cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
+  RefineParams P = base_params(cfg, pl);
+  auto exp_of = [&](int d, int s) -> uint64_t {
+    size_t k = (size_t)d * CPL_NUM_FAMILIES + s;
+    return k < expect.size() && expect[k] ? expect[k] : pl.cap;
+  };
+  // round 0 -> buffer 1
+  P.depth = 0;
+  for (int s = 0; s < pl.nf; ++s) {
+    P.fam_slot = s;
+    Worklist none{};
+    if (hier) {
+      // live root items go to buffer 0, evaluated from there into buffer 1
+      RefineParams T = P;
+      T.out = region(g.wl[0], pl.cap, s);
+      T.out_count = counter(2, s);
+      dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
+      P.in_count = counter(2, s);
+      P.out = region(g.wl[1], pl.cap, s);
+      P.out_count = counter(1, s);
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(0, s), pl), pl.block, P,
+                                    region(g.wl[0], pl.cap, s), (uint64_t)0);
+      P.in_count = nullptr;
+    } else {
+      P.out = region(g.wl[1], pl.cap, s);
+      P.out_count = counter(1, s);
+      dispatch_family<LaunchRoundB>(pl.fams[s], true, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
+    }
+  }
+  int cur = 1;
+  for (int d = 1; d <= cfg->max_depth; ++d) {
+    const int nxt = cur ^ 1;
+    P.depth = d;
+    fill(counter(nxt, 0), CPL_NUM_FAMILIES, 0u);
+    for (int s = 0; s < pl.nf; ++s) {
+      P.fam_slot = s;
+      P.in_count = counter(cur, s);
+      P.out = region(g.wl[nxt], pl.cap, s);
+      P.out_count = counter(nxt, s);
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(d, s), pl), pl.block, P,
+                                    region(g.wl[cur], pl.cap, s), (uint64_t)0);
+    }
+    cur = nxt;
+  }
+}
+
+int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
+  bool same = g_graph.valid && g_graph.flags == pl.flags && std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) == 0;
+  if (!same) {
+    g_graph.reset();
+    HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_all(cfg, pl, g_graph.expect);
+    hipError_t e = hipStreamEndCapture(g.stream, &g_graph.graph);
+    if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&g_graph.exec, g_graph.graph, nullptr, nullptr, 0));
+    g_graph.cfg = *cfg;
+    g_graph.flags = pl.flags;
+    g_graph.valid = true;
+  }
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventRecord(ev0, g.stream));
+  HIP_TRY(hipGraphLaunch(g_graph.exec, g.stream));
+  HIP_TRY(hipEventRecord(ev1, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  uint32_t ovf = 0;
+  HIP_TRY(hipMemcpy(&ovf, overflow_word(), sizeof(ovf), hipMemcpyDeviceToHost));
+  std::memset(stats, 0, sizeof(*stats));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  stats->seconds = ms * 1e-3;
+  stats->eval_seconds = stats->seconds;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (ovf) {
+    g_graph.reset();
+    return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
+  }
+  int rc = collect(pl, stats, root_flags);
+  if (rc) return rc;
+  // Record the round sizes of this replay; if they differ from the ones the
+  // graph's grids were sized for, re-capture on the next call.
+  std::vector<uint32_t> seen((size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES, 0);
+  for (int d = 0; d < stats->rounds; ++d)
+    for (int s = 0; s < pl.nf; ++s) {
+      uint64_t n = 0;
+      for (int v = 0; v < 8; ++v) n += stats->family_verdicts[d][pl.fams[s]][v];
+      seen[(size_t)d * CPL_NUM_FAMILIES + s] = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
+    }
+  if (seen != g_graph.expect) {
+    g_graph.expect = seen;
+    g_graph.valid = false;  // keep the exec until the next capture replaces it
+  }
+  return CPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!cfg || !stats) return fail(CPL_ERR_ARG, "cpl_refine_run: null cfg or stats");
+  Plan pl;
+  if ((rc = make_plan(cfg, root_flags != nullptr, pl))) return rc;
+  if ((rc = ensure_buffers(pl))) return rc;
+  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_HIER_ROOTS | CPL_OPT_NO_DIGEST;
+  if (cfg->options & ~known) return fail(CPL_ERR_ARG, "unsupported option bits");
+  if (cfg->options & CPL_OPT_GRAPH) return refine_graph(cfg, pl, stats, root_flags);
+  if (cfg->options & CPL_OPT_HIER_ROOTS) return fail(CPL_ERR_ARG, "CPL_OPT_HIER_ROOTS needs CPL_OPT_GRAPH");
+  return refine_plain(cfg, pl, stats, root_flags);
+}
+
+int cpl_autotune(cpl_refine_cfg* cfg, int reps, cpl_tune_point* points, int* n_points) {
+  if (!cfg || reps < 1) return fail(CPL_ERR_ARG, "cpl_autotune: bad argument");
+  static const int blocks[] = {64, 128, 256, 512, 1024};
+  static const int cpts[] = {1, 2, 4, 8};
+  cpl_refine_stats st;
+  double best = 1e30;
+  int bb = 256, bc = 1, k = 0;
+  for (int b : blocks) {
+    for (int c : cpts) {
+      cpl_refine_cfg t = *cfg;
+      t.block = b;
+      t.cells_per_thread = c;
+      int rc = cpl_refine_run(&t, &st, nullptr);  // warm-up (captures the graph)
+      if (rc) return rc;
+      rc = cpl_refine_run(&t, &st, nullptr);      // re-capture with measured round sizes
+      if (rc) return rc;
+      std::vector<double> times;
+      for (int r = 0; r < reps; ++r) {
+        if ((rc = cpl_refine_run(&t, &st, nullptr))) return rc;
+        times.push_back(st.seconds);
+      }
+      std::sort(times.begin(), times.end());
+      double med = times[times.size() / 2];
+      if (points && k < 20) points[k] = cpl_tune_point{b, c, med};
+      ++k;
+      if (med < best) { best = med; bb = b; bc = c; }
+    }
+  }
+  cfg->block = bb;
+  cfg->cells_per_thread = bc;
+  if (n_points) *n_points = std::min(k, 20);
   return CPL_OK;
 }
 

@@ -48,6 +48,8 @@ struct RefineParams {
   uint32_t* out_count;
   uint64_t out_cap;
   uint32_t* overflow;
+  const uint32_t* in_count;  // device-side item count of the input (graph mode), or null
+  int32_t digest;            // 1: accumulate the per-item digests
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {

@@ -93,8 +93,23 @@ typedef struct {
   int32_t max_depth;          /* number of refinement rounds after the root round (< CPL_MAX_ROUNDS) */
   int32_t split_mode;         /* 0: bisect the widest eligible axis; 1: bisect all eligible axes */
   uint32_t split_axes;        /* bitmask of eligible axes */
-  int32_t reserved[7];
+  uint32_t options;           /* CPL_OPT_* (0: the plain FP64 path, one launch per family and round) */
+  int32_t block;              /* threads per block of the evaluation kernels (0: 256) */
+  int32_t cells_per_thread;   /* items a thread takes per scheduling step (0: 1) */
+  int32_t reserved[4];
 } cpl_refine_cfg;
+
+/* Execution options of cpl_refine_run.  Every option changes scheduling or
+ * precision of intermediate filtering only: verdicts, counts and root flags are
+ * identical to the plain path. */
+enum {
+  CPL_OPT_GRAPH = 1,       /* all rounds enqueued with device-side sizes, captured once as a hipGraph, replayed */
+  CPL_OPT_PERSISTENT = 2,  /* one persistent kernel drains a mixed-depth queue (LDS-staged batches) */
+  CPL_OPT_HIER_ROOTS = 4,  /* decide admissibility once per (la, r1, r2) root triple */
+  CPL_OPT_FP32 = 8,        /* FP32 outward-rounded prefilter; FP64 certifies what it leaves undecided */
+  CPL_OPT_LEARNED = 16,    /* MLP proposal orders criteria / split axes (the interval check decides) */
+  CPL_OPT_NO_DIGEST = 256  /* skip the per-item digests (timing runs) */
+};
 
 /* Per (root cell, family) outcome flags. */
 enum {
@@ -161,6 +176,18 @@ int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* ver
  * device.  root_flags (may be NULL): n_roots * popcount(family_mask) bytes,
  * CPL_RF_* per (root, family) in increasing family order. */
 int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* root_flags);
+
+/* ---- autotuner ------------------------------------------------------------ */
+typedef struct {
+  int32_t block, cells_per_thread;
+  double seconds;  /* median device time of the run with this shape */
+} cpl_tune_point;
+
+/* Times cfg (with its options) for every block size in {64,128,256,512,1024}
+ * and cells-per-thread in {1,2,4,8} (median of `reps` runs each, after one
+ * warm-up), writes the fastest shape into cfg->block / cfg->cells_per_thread,
+ * and returns the table in `points` (up to 20 entries; may be NULL). */
+int cpl_autotune(cpl_refine_cfg* cfg, int reps, cpl_tune_point* points, int* n_points);
 
 /* ---- diagnostics (no reference counterpart; used by the parity tests) --- */
 /* Elementwise primitive op on n host doubles: 0 add_rd 1 add_ru 2 sub_rd 3 sub_ru

@@ -1,0 +1,64 @@
+"""Batched execution forms (graph-captured rounds, hierarchical roots, block
+shapes) give results identical to the plain path and to the CPU oracle."""
+import numpy as np
+import pytest
+
+from circlepacking_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+G, H = abi.OPT_GRAPH, abi.OPT_HIER_ROOTS
+
+
+def same(a, b):
+    sa, fa = a
+    sb, fb = b
+    assert sa.as_dict() == sb.as_dict()
+    assert np.array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("opts", [G, G | H])
+def test_graph_replays_match_plain_and_oracle(cpl, cplo, opts):
+    base = abi.cfg_4param(max_depth=6, grid=(16, 16, 16, 16))
+    plain = cpl.refine_run(base)
+    ref = cplo.refine_run(base)
+    same(plain, ref)
+    cfg = abi.with_options(base, opts)
+    for rep in range(4):  # capture, re-capture with measured sizes, then replays
+        same(cpl.refine_run(cfg), plain)
+        # eager work and a synchronisation between replays
+        cpl.prim("add_rd", np.arange(1000.0), 1.0)
+
+
+@pytest.mark.parametrize("block,cpt", [(64, 1), (128, 8), (512, 2), (1024, 4)])
+def test_block_shapes(cpl, block, cpt):
+    base = abi.cfg_small_2param(raw=True, n=128)
+    plain = cpl.refine_run(base)
+    for opts in (G, G | H):
+        same(cpl.refine_run(abi.with_options(base, opts, block, cpt)), plain)
+
+
+def test_full_grid_hierarchical_roots_match_plain(cpl):
+    base = abi.cfg_4param()
+    plain = cpl.refine_run(base)
+    cfg = abi.with_options(base, G | H)
+    for _ in range(2):
+        same(cpl.refine_run(cfg), plain)
+
+
+def test_no_digest_keeps_counts(cpl):
+    base = abi.cfg_4param(grid=(16, 16, 16, 16))
+    a, _ = cpl.refine_run(base)
+    b, _ = cpl.refine_run(abi.with_options(base, G | H | abi.OPT_NO_DIGEST))
+    da, db = a.as_dict(), b.as_dict()
+    for k in ("evaluations", "verdicts", "family_verdicts", "certified", "unresolved"):
+        assert da[k] == db[k]
+    assert all(x == 0 for x in db["digest"])
+
+
+def test_autotune_picks_a_listed_shape(cpl):
+    cfg = abi.with_options(abi.cfg_4param(grid=(16, 16, 16, 16)), G | H | abi.OPT_NO_DIGEST)
+    table = cpl.autotune(cfg, reps=1)
+    assert len(table) == 20
+    best = min(table, key=lambda t: t[2])
+    assert (cfg.block, cfg.cells_per_thread) == best[:2]
