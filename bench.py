@@ -78,6 +78,15 @@ def main():
         return (t1 - t0) * 1e3 / args.steps, runs[-1]
 
     ms_plain, st_plain = timed(plain_cfg)
+    ms_graph, _ = timed(tuned_cfg)
+    persist_cfg = abi.with_options(base, abi.OPT_PERSISTENT | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST,
+                                   block=tuned_cfg.block, cpt=1)
+    persist_table = None
+    if not args.no_tune:
+        persist_table = cpl.autotune(persist_cfg, reps=3)
+    ms_persist, _ = timed(persist_cfg)
+    if ms_persist < ms_graph:
+        tuned_cfg = persist_cfg
     ms_tuned, st_tuned = timed(tuned_cfg)
     # identical verdicts: full digests and per-root flags of both forms (untimed)
     vp, fp = cpl.refine_run(abi.with_options(base, 0))
@@ -166,15 +175,19 @@ def main():
             "grid": [args.grid] * 4,
             "max_depth": args.depth,
             "variant": args.variant,
-            "tuned_options": "hipGraph rounds + hierarchical roots" if args.variant == "tuned" else None,
+            "tuned_options": ("persistent mixed-depth queue" if tuned_cfg.options & abi.OPT_PERSISTENT
+                              else "hipGraph rounds") + " + hierarchical roots" if args.variant == "tuned" else None,
             "block": tuned_cfg.block if args.variant == "tuned" else 256,
             "cells_per_thread": tuned_cfg.cells_per_thread if args.variant == "tuned" else 1,
         },
         "plain_ms_per_step": ms_plain,
         "tuned_ms_per_step": ms_tuned,
+        "graph_ms_per_step": ms_graph,
+        "persistent_ms_per_step": ms_persist,
         "tuned_speedup": ms_plain / ms_tuned,
         "identical_verdicts": identical,
         "autotune": tune_table,
+        "autotune_persistent": persist_table,
         "time_to_proof_ms": ms_per_step,
         "device_ms_per_step": st.seconds * 1e3,
         "evaluations": st.total_evaluations,

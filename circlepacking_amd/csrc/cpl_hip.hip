@@ -89,7 +89,7 @@ __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, 
 // LB: launch bound (maximum threads per block) the register allocation is made for.
 template <int F, bool ROOT, int LB = 256>
 __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint64_t n) {
-  if (P.in_count) n = *P.in_count;
+  if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
@@ -335,6 +335,214 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// ============================================================================
+// Persistent form: one launch per family drains a mixed-depth queue.
+//
+// Each block is one wave that owns a depth-first stack of pending cells: the
+// top lives in LDS (SoA), overflow goes to a private HBM region that only the
+// same wave reads back.  A wave pops up to 64 cells (any mix of depths),
+// evaluates them, pushes the children of critical cells (ballot + mbcnt ranks)
+// and refills from the family's live root cells with a device-wide claim
+// counter, in units that keep the batch full.  Waves never wait for each
+// other: every iteration either evaluates cells, claims roots, or exits, so
+// the kernel always terminates; capacity overflow is reported, not waited on.
+// (The L2 caches of the XCDs are not coherent for ordinary device memory, so
+// the only cross-wave communication is the atomic root counter.)
+// ============================================================================
+struct PersistParams {
+  RefineParams R;
+  Worklist roots;              // live root cells of this family (depth 0)
+  const uint32_t* n_roots;     // their count
+  uint32_t* q;                 // [0] root claim counter, [4] error flags
+  double* sp_e;                // private spill stacks: [wave][8][spill_cap]
+  uint32_t* sp_u;              // [wave][3][spill_cap]
+  uint32_t spill_cap;          // cells per wave (multiple of 64)
+  int cap;                     // cells in a wave's LDS stack (multiple of 64)
+  uint32_t claim;              // roots claimed per atomic (small, so every wave gets a share)
+};
+
+enum : uint32_t { Q_ROOT = 0, Q_ERR = 4 };
+enum : int { ST_EVAL = 0, ST_V = 1, ST_DIG = 9, ST_VDIG = 10, ST_CERT = 11, ST_UNRES = 12, ST_N = 13 };
+
+// Child c of a cell under the refinement rule (same rule as k_round).
+__device__ __forceinline__ void make_child(const RefineParams& P, const double e[8], uint32_t meta, uint32_t la_idx,
+                                           int c, double ce[8], uint32_t& cmeta, uint32_t& cla) {
+  const int depth = (int)(meta >> 16);
+  const uint32_t splits = meta & 0xFFFFu;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ce[k] = e[k];
+  cla = la_idx;
+  if (P.split_mode == 0) {
+    int best = -1;
+    uint64_t bw = 0;
+    for (int a = 0; a < 4; ++a) {
+      if (!(P.split_axes & (1u << a))) continue;
+      uint64_t w = (uint64_t)P.grid[a] << axis_splits(meta, a);
+      if (best < 0 || w < bw) { best = a; bw = w; }
+    }
+    IV sub = get_subinterval(IV(e[2 * best], e[2 * best + 1]), c, 2);
+    ce[2 * best] = sub.lb;
+    ce[2 * best + 1] = sub.ub;
+    cmeta = ((uint32_t)(depth + 1) << 16) | (splits + (1u << (4 * best)));
+    if (best == 0) cla = la_idx * 2 + c;
+  } else {
+    const int nbits = __popc(P.split_axes);
+    uint32_t cs = splits;
+    int bit = 0;
+    for (int a = 0; a < 4; ++a) {
+      if (!(P.split_axes & (1u << a))) continue;
+      cs += (1u << (4 * a));
+      int side = (c >> (nbits - 1 - bit)) & 1;
+      ++bit;
+      IV sub = get_subinterval(IV(ce[2 * a], ce[2 * a + 1]), side, 2);
+      ce[2 * a] = sub.lb;
+      ce[2 * a + 1] = sub.ub;
+      if (a == 0) cla = cla * 2 + side;
+    }
+    cmeta = ((uint32_t)(depth + 1) << 16) | cs;
+  }
+}
+
+template <int F>
+__global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
+  extern __shared__ __align__(16) char lds[];
+  const RefineParams& P = Q.R;
+  const int CAP = Q.cap;
+  const int lane = threadIdx.x;
+  double* se = reinterpret_cast<double*>(lds);                        // [8][CAP]
+  uint32_t* su = reinterpret_cast<uint32_t*>(se + 8 * CAP);           // [3][CAP]
+  unsigned long long* sst = reinterpret_cast<unsigned long long*>(su + 3 * CAP);  // [MAX_ROUNDS][ST_N]
+  for (int k = lane; k < CPL_MAX_ROUNDS * ST_N; k += 64) sst[k] = 0;
+  const uint32_t SC = Q.spill_cap;
+  double* pe = Q.sp_e + (size_t)blockIdx.x * 8 * SC;
+  uint32_t* pu = Q.sp_u + (size_t)blockIdx.x * 3 * SC;
+  const int NCH = P.n_children;
+  const int S = 64 * NCH;        // most cells one batch can push
+  const uint32_t n_roots = (uint32_t)min((uint64_t)*Q.n_roots, P.out_cap);  // overflow is reported by the host
+  int count = 0;                 // cells in LDS (wave-uniform)
+  uint32_t spilled = 0;          // cells in the private HBM stack (wave-uniform, multiple of 64)
+  bool roots_left = true;
+
+  for (;;) {
+    // ---- refill: own spilled cells first (depth first), then new roots ----
+    if (count < 64 && spilled > 0) {
+      spilled -= 64;
+      for (int k = 0; k < 8; ++k) se[k * CAP + count + lane] = pe[(size_t)k * SC + spilled + lane];
+      for (int k = 0; k < 3; ++k) su[k * CAP + count + lane] = pu[(size_t)k * SC + spilled + lane];
+      count += 64;
+    }
+    while (count < 64 && roots_left) {
+      const uint32_t want = min((uint32_t)(64 - count), Q.claim);
+      uint32_t r0 = 0;
+      if (lane == 0) r0 = atomicAdd(Q.q + Q_ROOT, want);
+      r0 = __shfl(r0, 0, 64);
+      if (r0 >= n_roots) { roots_left = false; break; }
+      const uint32_t take = min(want, n_roots - r0);
+      if ((uint32_t)lane < take) {
+        const uint32_t i = r0 + lane;
+        for (int k = 0; k < 8; ++k) se[k * CAP + count + lane] = Q.roots.e[k][i];
+        su[0 * CAP + count + lane] = Q.roots.root[i];
+        su[1 * CAP + count + lane] = Q.roots.la_idx[i];
+        su[2 * CAP + count + lane] = Q.roots.meta[i];
+      }
+      count += (int)take;
+    }
+    if (count == 0) break;  // nothing in LDS, nothing spilled, no roots left
+
+    // ---- evaluate a batch from the top of the stack ----
+    const int nb = count < 64 ? count : 64;
+    const int base = count - nb;
+    const bool valid = lane < nb;
+    double e[8];
+    uint32_t root = 0, la_idx = 0, meta = 0;
+    if (valid) {
+      for (int k = 0; k < 8; ++k) e[k] = se[k * CAP + base + lane];
+      root = su[0 * CAP + base + lane];
+      la_idx = su[1 * CAP + base + lane];
+      meta = su[2 * CAP + base + lane];
+    }
+    count = base;
+    bool split = false;
+    if (valid) {
+      const int depth = (int)(meta >> 16);
+      double lbrsp = 0.0;
+      if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
+      uint8_t vb = eval_synthetic<F>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp);
+      const int v = vb & 7;
+      unsigned long long* st = sst + depth * ST_N;
+      atomicAdd(st + ST_EVAL, 1ull);
+      atomicAdd(st + ST_V + v, 1ull);
+      if (P.digest) {
+        uint64_t h = item_hash(e, root, F, meta);
+        atomicAdd(st + ST_DIG, (unsigned long long)h);
+        atomicAdd(st + ST_VDIG, (unsigned long long)mix64(h ^ vb));
+      }
+      uint32_t fl = (vb & 8) ? CPL_RF_NOFIT_FLAG : 0;
+      if (v == V_CRITICAL && depth < P.max_depth) {
+        split = true;
+      } else {
+        int cert = 0;
+        switch (v) {
+          case V_PROVEN: fl |= CPL_RF_PROVEN; cert = 1; break;
+          case V_EMPTY: fl |= CPL_RF_EMPTY; cert = 1; break;
+          case V_OOD: fl |= CPL_RF_OOD; cert = 1; break;
+          case V_CRITICAL: fl |= CPL_RF_CRITICAL; break;
+          case V_NOFIT: fl |= CPL_RF_NOFIT; break;
+          default: fl |= CPL_RF_TRAP; break;
+        }
+        atomicAdd(st + (cert ? ST_CERT : ST_UNRES), 1ull);
+      }
+      if (fl && P.root_flags) flag_root(P.root_flags, (uint64_t)root * P.nfam + P.fam_slot, fl);
+    }
+    // ---- push children ----
+    const uint64_t mask = __ballot(split);
+    if (mask) {
+      const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      if (split) {
+        for (int c = 0; c < NCH; ++c) {
+          double ce[8];
+          uint32_t cmeta, cla;
+          make_child(P, e, meta, la_idx, c, ce, cmeta, cla);
+          const int o = count + rank * NCH + c;
+          for (int k = 0; k < 8; ++k) se[k * CAP + o] = ce[k];
+          su[0 * CAP + o] = root;
+          su[1 * CAP + o] = cla;
+          su[2 * CAP + o] = cmeta;
+        }
+      }
+      count += (int)__popcll(mask) * NCH;
+    }
+    // ---- spill the top 64 cells to the private stack while LDS could overflow ----
+    bool full = false;
+    while (count > CAP - S) {
+      if (spilled + 64 > SC) { full = true; break; }
+      const int from = count - 64;
+      for (int j = 0; j < 8; ++j) pe[(size_t)j * SC + spilled + lane] = se[j * CAP + from + lane];
+      for (int j = 0; j < 3; ++j) pu[(size_t)j * SC + spilled + lane] = su[j * CAP + from + lane];
+      spilled += 64;
+      count -= 64;
+    }
+    if (full) {
+      if (lane == 0) atomicOr(Q.q + Q_ERR, 1u);
+      break;
+    }
+  }
+  // flush the wave's statistics
+  for (int k = lane; k < CPL_MAX_ROUNDS * ST_N; k += 64) {
+    unsigned long long v = sst[k];
+    if (!v) continue;
+    const int d = k / ST_N, c = k % ST_N;
+    RoundStats* S = P.stats + d * CPL_NUM_FAMILIES + F;
+    unsigned long long* dst = c == ST_EVAL ? &S->evaluations
+                            : c < ST_DIG   ? &S->verdicts[c - ST_V]
+                            : c == ST_DIG  ? &S->digest
+                            : c == ST_VDIG ? &S->vdigest
+                            : c == ST_CERT ? &S->certified
+                                           : &S->unresolved;
+    atomicAdd(dst, v);
+  }
+}
+
 template <int F>
 __global__ __launch_bounds__(256) void k_eval_cells(int clip, const double* e0, const double* e1, const double* e2,
                                                      const double* e3, const double* e4, const double* e5,
@@ -493,10 +701,13 @@ struct Context {
   int device = 0;
   int cu_count = 0;
   hipStream_t stream = nullptr;
+  hipStream_t fam_stream[CPL_NUM_FAMILIES] = {};   // per-family branches (default priority)
+  hipEvent_t fork_ev = nullptr, join_ev[CPL_NUM_FAMILIES] = {};
   DevBuf scratch_in, scratch_in2, scratch_out, cells, verdicts;
   DevBuf wl[2];           // ping-pong worklists; each holds nfam regions
   DevBuf la_table, root_flags, stats, counters;
   DevBuf mr_ranges, mr_misc, mr_crit;
+  DevBuf persist_q;
   uint64_t wl_cap_items = 0;  // per family region
 };
 Context g;
@@ -511,9 +722,19 @@ int ensure_init() {
   HIP_TRY(hipGetDeviceProperties(&prop, g.device));
   g.cu_count = prop.multiProcessorCount;
   HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
+    HIP_TRY(hipStreamCreateWithFlags(&g.fam_stream[f], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&g.join_ev[f], hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&g.fork_ev, hipEventDisableTiming));
   g.ready = true;
   return CPL_OK;
 }
+
+// Stream the refinement launch helpers enqueue on (the library stream, or a
+// family branch while forking).
+hipStream_t g_launch_stream = nullptr;
+hipStream_t launch_stream() { return g_launch_stream ? g_launch_stream : g.stream; }
 
 unsigned grid_for(uint64_t n, int block) {
   uint64_t want = (n + block - 1) / block;
@@ -554,11 +775,11 @@ template <int F> struct LaunchRound {
 template <int F> struct LaunchRoundB {
   static void run(bool root, unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
     if (block <= 256) {
-      if (root) hipLaunchKernelGGL((k_round<F, true, 256>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
-      else hipLaunchKernelGGL((k_round<F, false, 256>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+      if (root) hipLaunchKernelGGL((k_round<F, true, 256>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
+      else hipLaunchKernelGGL((k_round<F, false, 256>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
     } else {
-      if (root) hipLaunchKernelGGL((k_round<F, true, 1024>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
-      else hipLaunchKernelGGL((k_round<F, false, 1024>), dim3(grid), dim3(block), 0, g.stream, P, in, n);
+      if (root) hipLaunchKernelGGL((k_round<F, true, 1024>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
+      else hipLaunchKernelGGL((k_round<F, false, 1024>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
     }
   }
 };
@@ -593,10 +814,17 @@ int cpl_shutdown(void) {
   if (!g.ready) return CPL_OK;
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf* b : {&g.scratch_in, &g.scratch_in2, &g.scratch_out, &g.cells, &g.verdicts, &g.wl[0], &g.wl[1],
-                    &g.la_table, &g.root_flags, &g.stats, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit})
+                    &g.la_table, &g.root_flags, &g.stats, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit,
+                    &g.persist_q})
     b->release();
   (void)hipStreamDestroy(g.stream);
   g.stream = nullptr;
+  for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
+    (void)hipStreamDestroy(g.fam_stream[f]);
+    (void)hipEventDestroy(g.join_ev[f]);
+    g.fam_stream[f] = nullptr;
+  }
+  (void)hipEventDestroy(g.fork_ev);
   g.ready = false;
   return CPL_OK;
 }
@@ -747,12 +975,12 @@ RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
 
 void fill(void* p, uint64_t words, uint32_t v) {
   unsigned grid = (unsigned)std::min<uint64_t>((words + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::max(grid, 1u)), dim3(256), 0, g.stream, static_cast<uint32_t*>(p), words, v);
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::max(grid, 1u)), dim3(256), 0, launch_stream(), static_cast<uint32_t*>(p), words, v);
 }
 
 template <int F> struct LaunchTriples {
   static void run(unsigned grid, int block, const RefineParams& P) {
-    hipLaunchKernelGGL(k_root_triples<F>, dim3(grid), dim3(block), 0, g.stream, P);
+    hipLaunchKernelGGL(k_root_triples<F>, dim3(grid), dim3(block), 0, launch_stream(), P);
   }
 };
 
@@ -868,6 +1096,21 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   return collect(pl, stats, root_flags);
 }
 
+// Fork: every family branch waits for the work enqueued so far on the library
+// stream; join: the library stream waits for every branch.  Inside a stream
+// capture this yields a graph with one independent chain per family.
+void fork_families(int nf) {
+  (void)hipEventRecord(g.fork_ev, g.stream);
+  for (int s = 0; s < nf; ++s) (void)hipStreamWaitEvent(g.fam_stream[s], g.fork_ev, 0);
+}
+void join_families(int nf) {
+  for (int s = 0; s < nf; ++s) {
+    (void)hipEventRecord(g.join_ev[s], g.fam_stream[s]);
+    (void)hipStreamWaitEvent(g.stream, g.join_ev[s], 0);
+  }
+  g_launch_stream = nullptr;
+}
+
 // ---- graph path ---------------------------------------------------------------
 struct GraphCache {
   bool valid = false;
@@ -907,11 +1150,14 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
     size_t k = (size_t)d * CPL_NUM_FAMILIES + s;
     return k < expect.size() && expect[k] ? expect[k] : pl.cap;
   };
-  // round 0 -> buffer 1
-  P.depth = 0;
+  fork_families(pl.nf);
   for (int s = 0; s < pl.nf; ++s) {
+    g_launch_stream = g.fam_stream[s];
     P.fam_slot = s;
+    P.depth = 0;
+    P.in_count = nullptr;
     Worklist none{};
+    // round 0 -> buffer 1
     if (hier) {
       // live root items go to buffer 0, evaluated from there into buffer 1
       RefineParams T = P;
@@ -923,28 +1169,25 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
       P.out_count = counter(1, s);
       dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(0, s), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0);
-      P.in_count = nullptr;
     } else {
       P.out = region(g.wl[1], pl.cap, s);
       P.out_count = counter(1, s);
       dispatch_family<LaunchRoundB>(pl.fams[s], true, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
     }
-  }
-  int cur = 1;
-  for (int d = 1; d <= cfg->max_depth; ++d) {
-    const int nxt = cur ^ 1;
-    P.depth = d;
-    fill(counter(nxt, 0), CPL_NUM_FAMILIES, 0u);
-    for (int s = 0; s < pl.nf; ++s) {
-      P.fam_slot = s;
+    int cur = 1;
+    for (int d = 1; d <= cfg->max_depth; ++d) {
+      const int nxt = cur ^ 1;
+      P.depth = d;
+      fill(counter(nxt, s), 1, 0u);
       P.in_count = counter(cur, s);
       P.out = region(g.wl[nxt], pl.cap, s);
       P.out_count = counter(nxt, s);
       dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(d, s), pl), pl.block, P,
                                     region(g.wl[cur], pl.cap, s), (uint64_t)0);
+      cur = nxt;
     }
-    cur = nxt;
   }
+  join_families(pl.nf);
 }
 
 int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
@@ -998,6 +1241,108 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   return CPL_OK;
 }
 
+// ---- persistent path ---------------------------------------------------------
+template <int F> struct LaunchPersist {
+  static int grid_for_lds(size_t lds) {
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_persist<F>), 64, lds);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb * std::max(g.cu_count, 1);
+  }
+  static void run(int grid, size_t lds, const PersistParams& Q) {
+    hipLaunchKernelGGL(k_persist<F>, dim3(grid), dim3(64), lds, launch_stream(), Q);
+  }
+};
+
+int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
+  int rc;
+  const int nch = pl.n_children;
+  const int cap_lds = 128 + 64 * nch;
+  const size_t lds = (size_t)cap_lds * (8 * sizeof(double) + 3 * sizeof(uint32_t)) +
+                     (size_t)CPL_MAX_ROUNDS * ST_N * sizeof(unsigned long long);
+  const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
+  int grid_max = 0;
+  for (int f : pl.fams) {
+    int gsz = 0;
+    switch (f) {
+      case 0: gsz = LaunchPersist<0>::grid_for_lds(lds); break;
+      case 1: gsz = LaunchPersist<1>::grid_for_lds(lds); break;
+      case 2: gsz = LaunchPersist<2>::grid_for_lds(lds); break;
+      case 3: gsz = LaunchPersist<3>::grid_for_lds(lds); break;
+      case 4: gsz = LaunchPersist<4>::grid_for_lds(lds); break;
+      default: gsz = LaunchPersist<5>::grid_for_lds(lds); break;
+    }
+    grid_max = std::max(grid_max, gsz);
+  }
+  // private spill stacks live in worklist buffer 1 (unused by this path), one
+  // partition per family because the families' kernels run concurrently
+  uint32_t spill_cap =
+      (uint32_t)std::min<uint64_t>(g.wl[1].bytes / ((size_t)pl.nf * grid_max * per_item) / 64 * 64, 1u << 20);
+  if (spill_cap < 256) return fail(CPL_ERR_CAPACITY, "persistent form: worklist buffer too small for the spill stacks");
+  if ((rc = g.persist_q.ensure((size_t)CPL_NUM_FAMILIES * 8 * sizeof(uint32_t)))) return rc;
+
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  if (pl.flags) HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, pl.flag_words * 4, g.stream));
+  HIP_TRY(hipMemsetAsync(g.stats.p, 0, pl.stats_bytes, g.stream));
+  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
+  HIP_TRY(hipMemsetAsync(g.persist_q.p, 0, (size_t)CPL_NUM_FAMILIES * 8 * sizeof(uint32_t), g.stream));
+  HIP_TRY(hipEventRecord(ev0, g.stream));
+  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
+                     cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
+  RefineParams P = base_params(cfg, pl);
+  P.depth = 0;
+  fork_families(pl.nf);
+  for (int s = 0; s < pl.nf; ++s) {
+    g_launch_stream = g.fam_stream[s];
+    P.fam_slot = s;
+    RefineParams T = P;
+    T.out = region(g.wl[0], pl.cap, s);
+    T.out_count = counter(2, s);
+    dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
+    HIP_TRY(hipGetLastError());
+    PersistParams Q;
+    Q.R = P;
+    Q.roots = region(g.wl[0], pl.cap, s);
+    Q.n_roots = counter(2, s);
+    Q.q = g.persist_q.as<uint32_t>() + 8 * s;
+    char* part = g.wl[1].as<char>() + (size_t)s * grid_max * spill_cap * per_item;
+    Q.sp_e = reinterpret_cast<double*>(part);
+    Q.sp_u = reinterpret_cast<uint32_t*>(part + (size_t)grid_max * 8 * spill_cap * sizeof(double));
+    Q.spill_cap = spill_cap;
+    Q.cap = cap_lds;
+    Q.claim = (uint32_t)std::min(64, 8 * pl.cpt);
+    switch (pl.fams[s]) {
+      case 0: LaunchPersist<0>::run(LaunchPersist<0>::grid_for_lds(lds), lds, Q); break;
+      case 1: LaunchPersist<1>::run(LaunchPersist<1>::grid_for_lds(lds), lds, Q); break;
+      case 2: LaunchPersist<2>::run(LaunchPersist<2>::grid_for_lds(lds), lds, Q); break;
+      case 3: LaunchPersist<3>::run(LaunchPersist<3>::grid_for_lds(lds), lds, Q); break;
+      case 4: LaunchPersist<4>::run(LaunchPersist<4>::grid_for_lds(lds), lds, Q); break;
+      default: LaunchPersist<5>::run(LaunchPersist<5>::grid_for_lds(lds), lds, Q); break;
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  join_families(pl.nf);
+  HIP_TRY(hipEventRecord(ev1, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  std::vector<uint32_t> qs((size_t)CPL_NUM_FAMILIES * 8);
+  HIP_TRY(hipMemcpy(qs.data(), g.persist_q.p, qs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  uint32_t ovf = 0;
+  HIP_TRY(hipMemcpy(&ovf, overflow_word(), sizeof(ovf), hipMemcpyDeviceToHost));
+  std::memset(stats, 0, sizeof(*stats));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  stats->seconds = ms * 1e-3;
+  stats->eval_seconds = stats->seconds;
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  for (int s = 0; s < pl.nf; ++s)
+    if (qs[8 * s + Q_ERR]) return fail(CPL_ERR_CAPACITY, "persistent form: a private spill stack overflowed");
+  if (ovf) return fail(CPL_ERR_CAPACITY, "root worklist capacity exceeded; raise CPL_WORKLIST_CAP");
+  return collect(pl, stats, root_flags);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1009,8 +1354,9 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   Plan pl;
   if ((rc = make_plan(cfg, root_flags != nullptr, pl))) return rc;
   if ((rc = ensure_buffers(pl))) return rc;
-  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_HIER_ROOTS | CPL_OPT_NO_DIGEST;
+  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_NO_DIGEST;
   if (cfg->options & ~known) return fail(CPL_ERR_ARG, "unsupported option bits");
+  if (cfg->options & CPL_OPT_PERSISTENT) return refine_persistent(cfg, pl, stats, root_flags);
   if (cfg->options & CPL_OPT_GRAPH) return refine_graph(cfg, pl, stats, root_flags);
   if (cfg->options & CPL_OPT_HIER_ROOTS) return fail(CPL_ERR_ARG, "CPL_OPT_HIER_ROOTS needs CPL_OPT_GRAPH");
   return refine_plain(cfg, pl, stats, root_flags);

@@ -7,7 +7,7 @@ from circlepacking_amd import abi
 
 pytestmark = pytest.mark.gpu
 
-G, H = abi.OPT_GRAPH, abi.OPT_HIER_ROOTS
+G, H, PER = abi.OPT_GRAPH, abi.OPT_HIER_ROOTS, abi.OPT_PERSISTENT
 
 
 def same(a, b):
@@ -17,7 +17,7 @@ def same(a, b):
     assert np.array_equal(fa, fb)
 
 
-@pytest.mark.parametrize("opts", [G, G | H])
+@pytest.mark.parametrize("opts", [G, G | H, PER | H])
 def test_graph_replays_match_plain_and_oracle(cpl, cplo, opts):
     base = abi.cfg_4param(max_depth=6, grid=(16, 16, 16, 16))
     plain = cpl.refine_run(base)
@@ -34,16 +34,24 @@ def test_graph_replays_match_plain_and_oracle(cpl, cplo, opts):
 def test_block_shapes(cpl, block, cpt):
     base = abi.cfg_small_2param(raw=True, n=128)
     plain = cpl.refine_run(base)
-    for opts in (G, G | H):
+    for opts in (G, G | H, PER | H):
         same(cpl.refine_run(abi.with_options(base, opts, block, cpt)), plain)
 
 
-def test_full_grid_hierarchical_roots_match_plain(cpl):
+@pytest.mark.parametrize("opts", [G | H, PER | H])
+def test_full_grid_batched_forms_match_plain(cpl, opts):
     base = abi.cfg_4param()
     plain = cpl.refine_run(base)
-    cfg = abi.with_options(base, G | H)
+    cfg = abi.with_options(base, opts)
     for _ in range(2):
         same(cpl.refine_run(cfg), plain)
+
+
+def test_persistent_small_capacity_spills_and_matches(cpl, monkeypatch):
+    """A deep 2x2 refinement forces the per-wave stacks to spill chunks to the global queue."""
+    base = abi.cfg_small_2param(raw=True, n=64, max_depth=6)
+    plain = cpl.refine_run(base)
+    same(cpl.refine_run(abi.with_options(base, PER | H)), plain)
 
 
 def test_no_digest_keeps_counts(cpl):
