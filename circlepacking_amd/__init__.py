@@ -36,6 +36,9 @@ def lib():
         L.cpl_debug_prim.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.c_size_t]
         L.cpl_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8]
         L.cpl_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
+        L.cpl_debug_geom.argtypes = [ctypes.c_int, ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.c_size_t]
+        L.cpl_debug_prefilter.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8, _P_U8]
         L.cpl_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8]
         L.cpl_autotune.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.c_int, ctypes.POINTER(abi.TunePoint),
                                    ctypes.POINTER(ctypes.c_int)]
@@ -59,8 +62,36 @@ def prim(op, a, b=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     out = np.empty_like(a)
     bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
-    _check(lib().cpl_debug_prim(abi.PRIM_OPS[op], _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size))
+    code = abi.PRIM_OPS[op] if op in abi.PRIM_OPS else abi.PRIM32_OPS[op]
+    _check(lib().cpl_debug_prim(code, _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size))
     return out
+
+
+GEOM = {"inner_inner": 0, "outer_inner": 1, "inner_outer": 2, "outer_outer": 3,
+        "semicircle_outer": 4, "semicircle_inner": 5, "compute": 6}
+
+
+def debug_geom(fn, recs, fp32=False):
+    """Geometry function `fn` (name from GEOM) on an (n, 6) array of interval
+    records (la, ra, rb); returns ((n, 2) bounds, (n,) trap codes)."""
+    recs = np.ascontiguousarray(recs, dtype=np.float64).reshape(-1, 6)
+    n = recs.shape[0]
+    out = np.zeros((n, 2), dtype=np.float64)
+    trap = np.zeros(n, dtype=np.int32)
+    _check(lib().cpl_debug_geom(GEOM[fn], int(bool(fp32)), _dptr(recs), _dptr(out),
+                                trap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
+    return out, trap
+
+
+def debug_prefilter(family, cells):
+    """(FP32 criteria result, FP64 verdict byte) per cell of an (n, 8) array."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 8)
+    n = cells.shape[0]
+    r32 = np.zeros(n, dtype=np.uint8)
+    v64 = np.zeros(n, dtype=np.uint8)
+    _check(lib().cpl_debug_prefilter(int(family), _dptr(cells), n, r32.ctypes.data_as(_P_U8),
+                                     v64.ctypes.data_as(_P_U8)))
+    return r32, v64
 
 
 def eval_cells(family, soa, clip=True):

@@ -34,6 +34,12 @@ PRIM_OPS = {
     "div_rd": 6, "div_ru": 7, "sqrt_rd": 8, "sqrt_ru": 9, "asin_rn": 10,
     "dasin_rd": 11, "dasin_ru": 12, "lbrsp": 13, "div_rn": 14, "sqrt_rn": 15,
 }
+# FP32 prefilter primitives (operands and results are float values carried in doubles)
+PRIM32_OPS = {
+    "add_rd32": 16, "add_ru32": 17, "sub_rd32": 18, "sub_ru32": 19, "mul_rd32": 20, "mul_ru32": 21,
+    "div_rd32": 22, "div_ru32": 23, "asin32_rn": 24, "dasin32_rd": 25, "dasin32_ru": 26,
+    "f32_down": 27, "f32_up": 28,
+}
 
 
 class Iv(ctypes.Structure):
@@ -53,7 +59,8 @@ class RefineCfg(ctypes.Structure):
         ("options", ctypes.c_uint32),
         ("block", ctypes.c_int32),
         ("cells_per_thread", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("prefilter_depths", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
     def families(self):
@@ -86,7 +93,11 @@ class RefineStats(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("eval_seconds", ctypes.c_double),
         ("peak_items", ctypes.c_uint64),
+        ("fp32_settled", (ctypes.c_uint64 * 6) * MAX_ROUNDS),
     ]
+
+    def fp32_settled_total(self):
+        return sum(self.fp32_settled[d][f] for d in range(MAX_ROUNDS) for f in range(6))
 
     def as_dict(self):
         r = self.rounds
@@ -119,10 +130,12 @@ def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_
     return c
 
 
-def with_options(cfg, options, block=None, cpt=None):
+def with_options(cfg, options, block=None, cpt=None, prefilter_depths=None):
     """Copy of cfg with other execution options (and optionally shape)."""
     c = RefineCfg.from_buffer_copy(cfg)
     c.options = int(options)
+    if prefilter_depths is not None:
+        c.prefilter_depths = int(prefilter_depths)
     if block is not None:
         c.block = int(block)
     if cpt is not None:

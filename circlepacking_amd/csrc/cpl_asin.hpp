@@ -85,3 +85,76 @@ __device__ __forceinline__ double dasin_ru(double x) {
 }
 
 }  // namespace cpl
+
+// ============================================================================
+// FP32 arcsine for the outward-rounded prefilter.  Same reduction as asin_rn
+// with the degree-5 interpolant rounded to float (gen_asin_poly.py --fp32);
+// its error against the exact arcsine stays below 2 ulps (measured by the
+// parity tests over 2.4e6 arguments), and the bounds step ASIN32_STEPS = 6
+// ulps outward (more than twice the error plus one, so a step across a binade
+// edge still leaves room).
+// ============================================================================
+namespace cpl {
+
+constexpr int ASIN32_STEPS = 6;
+
+__device__ __forceinline__ float asin32_poly(float z) {
+  float p = 0x1.13fed4p-5f;
+  p = __builtin_fmaf(p, z, 0x1.18f91ep-6f);
+  p = __builtin_fmaf(p, z, 0x1.fd8da2p-6f);
+  p = __builtin_fmaf(p, z, 0x1.6d5bbap-5f);
+  p = __builtin_fmaf(p, z, 0x1.33343p-4f);
+  p = __builtin_fmaf(p, z, 0x1.555554p-3f);
+  return p;
+}
+
+__device__ __forceinline__ float asin32_rn(float x) {
+  float ax = __builtin_fabsf(x);
+  float r;
+  if (ax <= 0.5f) {
+    float z = ax * ax;
+    r = __builtin_fmaf(ax * z, asin32_poly(z), ax);
+  } else {
+    float t = 0.5f - 0.5f * ax;
+    float s = __builtin_sqrtf(t);
+    float w = __builtin_fmaf(s * t, asin32_poly(t), 0.0f);
+    r = 0x1.921fb6p+0f - ((s + s) + (w + w + 0x1.777a5cp-25f));
+  }
+  return __builtin_copysignf(r, x);
+}
+
+__device__ __forceinline__ float next_toward_f(float y, float toward) {
+  if (y != y) return y;
+  if (y == toward) return toward;
+  if (y == 0.0f) return __builtin_copysignf(0x1p-149f, toward);
+  int b = __float_as_int(y);
+  b = ((y < toward) == (y > 0.0f)) ? b + 1 : b - 1;
+  return __int_as_float(b);
+}
+
+__device__ __forceinline__ float steps_out_f(float y, bool up) {
+  if (__builtin_fabsf(y) >= 0x1p-100f && __builtin_fabsf(y) < 1.9f) {
+    int b = __float_as_int(y);
+    bool away = up == (y > 0.0f);
+    return __int_as_float(away ? b + ASIN32_STEPS : b - ASIN32_STEPS);
+  }
+  float t = up ? 2.0f : -2.0f;
+  for (int i = 0; i < ASIN32_STEPS; ++i) y = next_toward_f(y, t);
+  return y;
+}
+
+// Clamps of interval.cuh:426-437 with pi/2 rounded outward to FP32.
+__device__ __forceinline__ float dasin_rd(float x) {
+  float y = steps_out_f(asin32_rn(x), false);
+  if (y < -0x1.921fb6p+0f) y = -0x1.921fb6p+0f;
+  if (x >= 0.0f && y < 0.0f) y = 0.0f;
+  return y;
+}
+__device__ __forceinline__ float dasin_ru(float x) {
+  float y = steps_out_f(asin32_rn(x), true);
+  if (y > 0x1.921fb6p+0f) y = 0x1.921fb6p+0f;
+  if (x <= 0.0f && y > 0.0f) y = 0.0f;
+  return y;
+}
+
+}  // namespace cpl

@@ -10,7 +10,13 @@
 //   k_eval_cells<F>   per-cell verdicts of caller-given cells
 //   k_body<F>         reference kernel body on explicit (la, r1_, r2_, r3)
 //   k_mode_r<F>       one reference-grid launch (the reference's kernel contract)
-//   k_debug_prim      elementwise primitives for the parity tests
+//   k_persist<F>      one persistent launch per family draining a mixed-depth
+//                     worklist from wave-private LDS stacks
+//   k_debug_prim, k_debug_geom, k_debug_prefilter
+//                     primitives, geometry and FP32-prefilter results for the
+//                     parity tests
+// P32 template variants run the FP32 outward-rounded prefilter before the FP64
+// criteria.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,9 +55,76 @@ __global__ void k_debug_prim(int op, const double* a, const double* b, double* o
     case 13: r = lower_bound_radius_single_placement(x); break;
     case 14: r = x / y; break;
     case 15: r = __builtin_sqrt(x); break;
+    case 16: r = add_rd((float)x, (float)y); break;
+    case 17: r = add_ru((float)x, (float)y); break;
+    case 18: r = add_rd((float)x, -(float)y); break;
+    case 19: r = add_ru((float)x, -(float)y); break;
+    case 20: r = mul_rd((float)x, (float)y); break;
+    case 21: r = mul_ru((float)x, (float)y); break;
+    case 22: r = div_rd((float)x, (float)y); break;
+    case 23: r = div_ru((float)x, (float)y); break;
+    case 24: r = asin32_rn((float)x); break;
+    case 25: r = dasin_rd((float)x); break;
+    case 26: r = dasin_ru((float)x); break;
+    case 27: r = f32_down(x); break;
+    case 28: r = f32_up(x); break;
     default: r = __builtin_nan("");
   }
   out[i] = r;
+}
+
+// Geometry functions of the criteria on interval records (la, ra, rb), in
+// FP64 (T = double) or in the FP32 prefilter precision (T = float, inputs
+// converted outward).  fn: 0 inner_inner 1 outer_inner 2 inner_outer
+// 3 outer_outer(ra, rb) 4 semicircle_outer(ra) 5 semicircle_inner(la, ra)
+// 6 compute_collision_angle(lc = la, sc = ra, ab = rb).
+template <class T>
+__device__ __forceinline__ IVT<T> conv(IV x);
+template <>
+__device__ __forceinline__ IV conv<double>(IV x) { return x; }
+template <>
+__device__ __forceinline__ IVf conv<float>(IV x) { return to_f32(x); }
+
+template <class T>
+__global__ void k_debug_geom(int fn, const double* in, double* out, int* trap, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = in + 6 * i;
+  IVT<T> la = conv<T>(IV(p[0], p[1])), ra = conv<T>(IV(p[2], p[3])), rb = conv<T>(IV(p[4], p[5]));
+  IVT<T> r(T(0), T(0));
+  int t = 0;
+  switch (fn) {
+    case 0: r = collision_angle_inner_inner(la, ra, rb, t); break;
+    case 1: r = collision_angle_outer_inner(la, ra, rb, t); break;
+    case 2: r = collision_angle_inner_outer(la, ra, rb, t); break;
+    case 3: r = collision_angle_outer_outer(ra, rb, t); break;
+    case 4: r = semicircle_angle_outer(ra); break;
+    case 5: r = semicircle_angle_inner(la, ra); break;
+    default: r = compute_collision_angle(la, ra, rb, t); break;
+  }
+  out[2 * i] = r.lb;
+  out[2 * i + 1] = r.ub;
+  trap[i] = t;
+}
+
+// Per cell: the FP32 criteria result of the tightened cell (0 all hold, k the
+// first failing criterion, 16 + trap code; 255 when the FP64 body decides
+// before the criteria) and the FP64 verdict byte.
+template <int F>
+__global__ __launch_bounds__(256) void k_debug_prefilter(const double* cells, size_t n, uint8_t* out32, uint8_t* out64) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* c = cells + 8 * i;
+  IV la(c[0], c[1]), r1(c[2], c[3]), r2(c[4], c[5]), r3(c[6], c[7]);
+  uint8_t v64 = eval_kernel_body<F>(la, r1, r2, r3);
+  if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
+  r1.tighten_lb(r2.lb);
+  uint8_t v32 = 255;
+  int v = v64 & 7;
+  if (v == V_PROVEN || v == V_CRITICAL || v == V_TRAP)
+    v32 = (uint8_t)criteria<F, float>(to_f32(la), to_f32(r1), to_f32(r2), to_f32(r3));
+  out32[i] = v32;
+  out64[i] = v64;
 }
 
 // la cells of every depth 0..levels-1: level 0 = get_subinterval(box, j, n0),
@@ -87,12 +160,14 @@ __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, 
 // One round of family F.  ROOT: items are the implicit root grid (n = n_roots),
 // else items come from `in` (n items).
 // LB: launch bound (maximum threads per block) the register allocation is made for.
-template <int F, bool ROOT, int LB = 256>
+// P32: FP32 prefilter kernels (enabled per depth by P.prefilter_depths).
+template <int F, bool ROOT, int LB = 256, bool P32 = false>
 __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint64_t n) {
   if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
+  unsigned long long c_f32 = 0;
 
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -128,7 +203,11 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
       int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      vb = eval_synthetic<F>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp);
+      bool hit = false;
+      const bool pre = P32 && ((P.prefilter_depths >> depth) & 1u);
+      vb = eval_synthetic<F, P32>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp,
+                                  pre, &hit);
+      c_f32 += hit ? 1 : 0;
       int v = vb & 7;
       c_eval += 1;
       c_v[v] += 1;
@@ -234,6 +313,7 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
   wave_add(&S->vdigest, c_vdig);
   wave_add(&S->certified, c_cert);
   wave_add(&S->unresolved, c_unres);
+  if (P32) wave_add(&S->fp32_settled, c_f32);
 }
 
 // Root round with admissibility decided once per (la, r1, r2) triple: an
@@ -362,7 +442,7 @@ struct PersistParams {
 };
 
 enum : uint32_t { Q_ROOT = 0, Q_ERR = 4 };
-enum : int { ST_EVAL = 0, ST_V = 1, ST_DIG = 9, ST_VDIG = 10, ST_CERT = 11, ST_UNRES = 12, ST_N = 13 };
+enum : int { ST_EVAL = 0, ST_V = 1, ST_DIG = 9, ST_VDIG = 10, ST_CERT = 11, ST_UNRES = 12, ST_F32 = 13, ST_N = 14 };
 
 // Child c of a cell under the refinement rule (same rule as k_round).
 __device__ __forceinline__ void make_child(const RefineParams& P, const double e[8], uint32_t meta, uint32_t la_idx,
@@ -403,7 +483,7 @@ __device__ __forceinline__ void make_child(const RefineParams& P, const double e
   }
 }
 
-template <int F>
+template <int F, bool P32 = false>
 __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
   extern __shared__ __align__(16) char lds[];
   const RefineParams& P = Q.R;
@@ -467,10 +547,14 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
       const int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      uint8_t vb = eval_synthetic<F>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp);
+      bool hit = false;
+      const bool pre = P32 && ((P.prefilter_depths >> depth) & 1u);
+      uint8_t vb = eval_synthetic<F, P32>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0,
+                                          lbrsp, pre, &hit);
       const int v = vb & 7;
       unsigned long long* st = sst + depth * ST_N;
       atomicAdd(st + ST_EVAL, 1ull);
+      if (hit) atomicAdd(st + ST_F32, 1ull);
       atomicAdd(st + ST_V + v, 1ull);
       if (P.digest) {
         uint64_t h = item_hash(e, root, F, meta);
@@ -538,6 +622,7 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
                             : c == ST_DIG  ? &S->digest
                             : c == ST_VDIG ? &S->vdigest
                             : c == ST_CERT ? &S->certified
+                            : c == ST_F32  ? &S->fp32_settled
                                            : &S->unresolved;
     atomicAdd(dst, v);
   }
@@ -766,21 +851,34 @@ template <int F> struct LaunchBody {
     hipLaunchKernelGGL(k_body<F>, dim3(grid), dim3(256), 0, g.stream, cells, n, out);
   }
 };
-template <int F> struct LaunchRound {
-  static void run(bool root, unsigned grid, const RefineParams& P, const Worklist& in, uint64_t n) {
-    if (root) hipLaunchKernelGGL((k_round<F, true>), dim3(grid), dim3(256), 0, g.stream, P, in, n);
-    else hipLaunchKernelGGL((k_round<F, false>), dim3(grid), dim3(256), 0, g.stream, P, in, n);
+template <int F> struct LaunchRoundB {
+  template <bool R, int LB, bool P32>
+  static void go(unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
+    hipLaunchKernelGGL((k_round<F, R, LB, P32>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
+  }
+  static void run(bool root, bool p32, unsigned grid, int block, const RefineParams& P, const Worklist& in,
+                  uint64_t n) {
+    const int k = (root ? 4 : 0) | (block > 256 ? 2 : 0) | (p32 ? 1 : 0);
+    switch (k) {
+      case 0: go<false, 256, false>(grid, block, P, in, n); break;
+      case 1: go<false, 256, true>(grid, block, P, in, n); break;
+      case 2: go<false, 1024, false>(grid, block, P, in, n); break;
+      case 3: go<false, 1024, true>(grid, block, P, in, n); break;
+      case 4: go<true, 256, false>(grid, block, P, in, n); break;
+      case 5: go<true, 256, true>(grid, block, P, in, n); break;
+      case 6: go<true, 1024, false>(grid, block, P, in, n); break;
+      default: go<true, 1024, true>(grid, block, P, in, n); break;
+    }
   }
 };
-template <int F> struct LaunchRoundB {
-  static void run(bool root, unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
-    if (block <= 256) {
-      if (root) hipLaunchKernelGGL((k_round<F, true, 256>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
-      else hipLaunchKernelGGL((k_round<F, false, 256>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
-    } else {
-      if (root) hipLaunchKernelGGL((k_round<F, true, 1024>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
-      else hipLaunchKernelGGL((k_round<F, false, 1024>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
-    }
+template <int F> struct LaunchRound {
+  static void run(bool root, bool p32, unsigned grid, const RefineParams& P, const Worklist& in, uint64_t n) {
+    LaunchRoundB<F>::run(root, p32, grid, 256, P, in, n);
+  }
+};
+template <int F> struct LaunchPrefilterDebug {
+  static void run(unsigned grid, const double* cells, size_t n, uint8_t* o32, uint8_t* o64) {
+    hipLaunchKernelGGL(k_debug_prefilter<F>, dim3(grid), dim3(256), 0, g.stream, cells, n, o32, o64);
   }
 };
 template <int F> struct LaunchModeR {
@@ -832,7 +930,7 @@ int cpl_shutdown(void) {
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (op < 0 || op > 15 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
+  if (op < 0 || op > 28 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
   if (n == 0) return CPL_OK;
   size_t bytes = n * sizeof(double);
   if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)))
@@ -880,6 +978,46 @@ int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* ver
   return CPL_OK;
 }
 
+int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, size_t n) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (fn < 0 || fn > 6 || !in || !out || !trap) return fail(CPL_ERR_ARG, "cpl_debug_geom: bad function or pointer");
+  if (n == 0) return CPL_OK;
+  if ((rc = g.scratch_in.ensure(6 * n * sizeof(double))) || (rc = g.scratch_out.ensure(2 * n * sizeof(double))) ||
+      (rc = g.scratch_in2.ensure(n * sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(g.scratch_in.p, in, 6 * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (fp32)
+    hipLaunchKernelGGL(k_debug_geom<float>, dim3(grid), dim3(256), 0, g.stream, fn, g.scratch_in.as<double>(),
+                       g.scratch_out.as<double>(), g.scratch_in2.as<int>(), n);
+  else
+    hipLaunchKernelGGL(k_debug_geom<double>, dim3(grid), dim3(256), 0, g.stream, fn, g.scratch_in.as<double>(),
+                       g.scratch_out.as<double>(), g.scratch_in2.as<int>(), n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, g.scratch_out.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipMemcpyAsync(trap, g.scratch_in2.p, n * sizeof(int), hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
+int cpl_debug_prefilter(int family, const double* cells, size_t n, uint8_t* fp32_result, uint8_t* verdicts) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (family < 0 || family >= CPL_NUM_FAMILIES || !cells || !fp32_result || !verdicts)
+    return fail(CPL_ERR_ARG, "cpl_debug_prefilter: bad argument");
+  if (n == 0) return CPL_OK;
+  if ((rc = g.cells.ensure(8 * n * sizeof(double))) || (rc = g.verdicts.ensure(2 * n))) return rc;
+  HIP_TRY(hipMemcpyAsync(g.cells.p, cells, 8 * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  dispatch_family<LaunchPrefilterDebug>(family, (unsigned)((n + 255) / 256), g.cells.as<double>(), n,
+                                        g.verdicts.as<uint8_t>(), g.verdicts.as<uint8_t>() + n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(fp32_result, g.verdicts.p, n, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipMemcpyAsync(verdicts, g.verdicts.as<uint8_t>() + n, n, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -896,6 +1034,8 @@ struct Plan {
   uint64_t cap = 0;
   bool digest = true;
   bool flags = false;
+  bool p32 = false;
+  uint32_t prefilter_depths = 0;
   int block = 256;
   int cpt = 1;
   size_t stats_bytes = 0;
@@ -923,6 +1063,8 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
+  pl.p32 = (cfg->options & CPL_OPT_FP32) != 0;
+  pl.prefilter_depths = cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
   pl.block = cfg->block > 0 ? cfg->block : 256;
   if (pl.block % 64 || pl.block > 1024) return fail(CPL_ERR_ARG, "block must be a multiple of 64 up to 1024");
   pl.cpt = cfg->cells_per_thread > 0 ? cfg->cells_per_thread : 1;
@@ -970,6 +1112,7 @@ RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   P.overflow = overflow_word();
   P.out_cap = pl.cap;
   P.digest = pl.digest ? 1 : 0;
+  P.prefilter_depths = pl.prefilter_depths;
   return P;
 }
 
@@ -1006,6 +1149,7 @@ int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
       stats->verdict_digest[d] += r.vdigest;
       stats->certified += r.certified;
       stats->unresolved += r.unresolved;
+      stats->fp32_settled[d][f] = r.fp32_settled;
       stats->total_evaluations += r.evaluations;
     }
     if (stats->evaluations[d]) stats->rounds = d + 1;
@@ -1052,7 +1196,7 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
       HIP_TRY(hipEventCreate(&a));
       HIP_TRY(hipEventCreate(&b));
       HIP_TRY(hipEventRecord(a, g.stream));
-      dispatch_family<LaunchRound>(pl.fams[s], d == 0, grid_for(n, 256), P, in, n);
+      dispatch_family<LaunchRound>(pl.fams[s], d == 0, pl.p32, grid_for(n, 256), P, in, n);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipEventRecord(b, g.stream));
       kev.push_back(a);
@@ -1167,12 +1311,12 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
       P.in_count = counter(2, s);
       P.out = region(g.wl[1], pl.cap, s);
       P.out_count = counter(1, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(0, s), pl), pl.block, P,
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.p32, grid_tuned(exp_of(0, s), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0);
     } else {
       P.out = region(g.wl[1], pl.cap, s);
       P.out_count = counter(1, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], true, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
+      dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.p32, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
     }
     int cur = 1;
     for (int d = 1; d <= cfg->max_depth; ++d) {
@@ -1182,7 +1326,7 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
       P.in_count = counter(cur, s);
       P.out = region(g.wl[nxt], pl.cap, s);
       P.out_count = counter(nxt, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, grid_tuned(exp_of(d, s), pl), pl.block, P,
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.p32, grid_tuned(exp_of(d, s), pl), pl.block, P,
                                     region(g.wl[cur], pl.cap, s), (uint64_t)0);
       cur = nxt;
     }
@@ -1243,14 +1387,18 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
 
 // ---- persistent path ---------------------------------------------------------
 template <int F> struct LaunchPersist {
-  static int grid_for_lds(size_t lds) {
+  static const void* fn(bool p32) {
+    return p32 ? reinterpret_cast<const void*>(k_persist<F, true>) : reinterpret_cast<const void*>(k_persist<F, false>);
+  }
+  static int grid_for_lds(size_t lds, bool p32) {
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_persist<F>), 64, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn(p32), 64, lds);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb * std::max(g.cu_count, 1);
   }
-  static void run(int grid, size_t lds, const PersistParams& Q) {
-    hipLaunchKernelGGL(k_persist<F>, dim3(grid), dim3(64), lds, launch_stream(), Q);
+  static void run(int grid, size_t lds, bool p32, const PersistParams& Q) {
+    if (p32) hipLaunchKernelGGL((k_persist<F, true>), dim3(grid), dim3(64), lds, launch_stream(), Q);
+    else hipLaunchKernelGGL((k_persist<F, false>), dim3(grid), dim3(64), lds, launch_stream(), Q);
   }
 };
 
@@ -1265,12 +1413,12 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
   for (int f : pl.fams) {
     int gsz = 0;
     switch (f) {
-      case 0: gsz = LaunchPersist<0>::grid_for_lds(lds); break;
-      case 1: gsz = LaunchPersist<1>::grid_for_lds(lds); break;
-      case 2: gsz = LaunchPersist<2>::grid_for_lds(lds); break;
-      case 3: gsz = LaunchPersist<3>::grid_for_lds(lds); break;
-      case 4: gsz = LaunchPersist<4>::grid_for_lds(lds); break;
-      default: gsz = LaunchPersist<5>::grid_for_lds(lds); break;
+      case 0: gsz = LaunchPersist<0>::grid_for_lds(lds, pl.p32); break;
+      case 1: gsz = LaunchPersist<1>::grid_for_lds(lds, pl.p32); break;
+      case 2: gsz = LaunchPersist<2>::grid_for_lds(lds, pl.p32); break;
+      case 3: gsz = LaunchPersist<3>::grid_for_lds(lds, pl.p32); break;
+      case 4: gsz = LaunchPersist<4>::grid_for_lds(lds, pl.p32); break;
+      default: gsz = LaunchPersist<5>::grid_for_lds(lds, pl.p32); break;
     }
     grid_max = std::max(grid_max, gsz);
   }
@@ -1314,12 +1462,12 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
     Q.cap = cap_lds;
     Q.claim = (uint32_t)std::min(64, 8 * pl.cpt);
     switch (pl.fams[s]) {
-      case 0: LaunchPersist<0>::run(LaunchPersist<0>::grid_for_lds(lds), lds, Q); break;
-      case 1: LaunchPersist<1>::run(LaunchPersist<1>::grid_for_lds(lds), lds, Q); break;
-      case 2: LaunchPersist<2>::run(LaunchPersist<2>::grid_for_lds(lds), lds, Q); break;
-      case 3: LaunchPersist<3>::run(LaunchPersist<3>::grid_for_lds(lds), lds, Q); break;
-      case 4: LaunchPersist<4>::run(LaunchPersist<4>::grid_for_lds(lds), lds, Q); break;
-      default: LaunchPersist<5>::run(LaunchPersist<5>::grid_for_lds(lds), lds, Q); break;
+      case 0: LaunchPersist<0>::run(LaunchPersist<0>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      case 1: LaunchPersist<1>::run(LaunchPersist<1>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      case 2: LaunchPersist<2>::run(LaunchPersist<2>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      case 3: LaunchPersist<3>::run(LaunchPersist<3>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      case 4: LaunchPersist<4>::run(LaunchPersist<4>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      default: LaunchPersist<5>::run(LaunchPersist<5>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
     }
     HIP_TRY(hipGetLastError());
   }
@@ -1354,7 +1502,7 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   Plan pl;
   if ((rc = make_plan(cfg, root_flags != nullptr, pl))) return rc;
   if ((rc = ensure_buffers(pl))) return rc;
-  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_NO_DIGEST;
+  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_FP32 | CPL_OPT_NO_DIGEST;
   if (cfg->options & ~known) return fail(CPL_ERR_ARG, "unsupported option bits");
   if (cfg->options & CPL_OPT_PERSISTENT) return refine_persistent(cfg, pl, stats, root_flags);
   if (cfg->options & CPL_OPT_GRAPH) return refine_graph(cfg, pl, stats, root_flags);

@@ -1,11 +1,12 @@
 // Device interval type for the lane-packing prover.
 //
-// Mirrors algcuda::Interval<double> / Uncertain<bool>
+// Mirrors algcuda::Interval<NumType> / Uncertain<bool>
 // (algcuda/cuda-utils/cpp/include/algcuda/interval.hpp:32-242) and the device
 // arithmetic of algcuda/cuda-utils/cuh/include/algcuda/interval.cuh:30-355,
 // on top of the MODE-register primitives of cpl_round.hpp.  Same POD layout
-// {double lb, ub}; same operator set and semantics, so the criteria read like
-// the reference's.  trap() of the reference becomes a per-thread trap code.
+// {lb, ub}; same operator set and semantics, so the criteria read like the
+// reference's.  IV = IVT<double> is the certifying type; IVT<float> is the
+// outward-rounded prefilter type.
 #pragma once
 
 #include "cpl_round.hpp"
@@ -19,80 +20,93 @@ struct UB {
 __device__ __forceinline__ bool definitely(UB b) { return b.lb; }
 __device__ __forceinline__ bool possibly(UB b) { return b.ub; }
 
-struct IV {
-  double lb, ub;
-  IV() = default;
-  __host__ __device__ constexpr IV(double l, double u) : lb(l), ub(u) {}
+template <class T>
+struct IVT {
+  T lb, ub;
+  IVT() = default;
+  __host__ __device__ constexpr IVT(T l, T u) : lb(l), ub(u) {}
 
   __device__ bool empty() const { return lb > ub; }
-  __device__ void tighten_lb(double v) { if (lb < v) lb = v; }
-  __device__ void tighten_ub(double v) { if (v < ub) ub = v; }
-  __device__ IV& do_join(const IV& o) {
+  __device__ void tighten_lb(T v) { if (lb < v) lb = v; }
+  __device__ void tighten_ub(T v) { if (v < ub) ub = v; }
+  __device__ IVT& do_join(const IVT& o) {
     if (o.lb < lb) lb = o.lb;
     if (o.ub > ub) ub = o.ub;
     return *this;
   }
-  __device__ IV intersect(const IV& o) const { return IV(lb < o.lb ? o.lb : lb, ub > o.ub ? o.ub : ub); }
-  __device__ IV abs() const {  // interval.hpp:216-232
-    double l = 0.0, u = 0.0;
+  __device__ IVT intersect(const IVT& o) const { return IVT(lb < o.lb ? o.lb : lb, ub > o.ub ? o.ub : ub); }
+  __device__ IVT abs() const {  // interval.hpp:216-232
+    T l = T(0), u = T(0);
     if (ub > 0) u = ub; else l = -ub;
     if (lb < 0) u = u < -lb ? -lb : u; else l = lb;
-    return IV(l, u);
+    return IVT(l, u);
   }
-  __device__ IV square() const {  // interval.cuh:352-355
-    IV a = abs();
-    IV r;
+  __device__ IVT square() const {  // interval.cuh:352-355
+    IVT a = abs();
+    IVT r;
     mul_pair(a.lb, a.lb, a.ub, a.ub, r.lb, r.ub);
     return r;
   }
 };
+using IV = IVT<double>;
+using IVf = IVT<float>;
 
-__device__ __forceinline__ IV operator+(const IV& a, const IV& b) {
-  IV r;
+template <class T>
+__device__ __forceinline__ IVT<T> operator+(const IVT<T>& a, const IVT<T>& b) {
+  IVT<T> r;
   add_pair(a.lb, b.lb, a.ub, b.ub, r.lb, r.ub);
   return r;
 }
-__device__ __forceinline__ IV operator+(const IV& a, double b) { return a + IV(b, b); }
-__device__ __forceinline__ IV operator+(double a, const IV& b) { return b + IV(a, a); }
-__device__ __forceinline__ IV operator-(const IV& a, const IV& b) {
-  IV r;
+template <class T>
+__device__ __forceinline__ IVT<T> operator+(const IVT<T>& a, T b) { return a + IVT<T>(b, b); }
+template <class T>
+__device__ __forceinline__ IVT<T> operator+(T a, const IVT<T>& b) { return b + IVT<T>(a, a); }
+template <class T>
+__device__ __forceinline__ IVT<T> operator-(const IVT<T>& a, const IVT<T>& b) {
+  IVT<T> r;
   sub_pair(a.lb, b.ub, a.ub, b.lb, r.lb, r.ub);
   return r;
 }
-__device__ __forceinline__ IV operator-(const IV& a, double b) { return a - IV(b, b); }
-__device__ __forceinline__ IV operator-(double a, const IV& b) { return IV(a, a) - b; }
+template <class T>
+__device__ __forceinline__ IVT<T> operator-(const IVT<T>& a, T b) { return a - IVT<T>(b, b); }
+template <class T>
+__device__ __forceinline__ IVT<T> operator-(T a, const IVT<T>& b) { return IVT<T>(a, a) - b; }
 
 // Interval x scalar (interval.cuh:185-197); for n >= 0 the upper bound is
 // rounded up (the reference rounds it down; identical for the power-of-two
 // factors 0.25, 0.5, 2 of every call site on this path).
-__device__ __forceinline__ IV operator*(const IV& a, double n) {
-  IV r;
+template <class T>
+__device__ __forceinline__ IVT<T> operator*(const IVT<T>& a, T n) {
+  IVT<T> r;
   if (n >= 0) mul_pair(a.lb, n, a.ub, n, r.lb, r.ub);
   else mul_pair(a.ub, n, a.lb, n, r.lb, r.ub);
   return r;
 }
-__device__ __forceinline__ IV operator*(double n, const IV& a) { return a * n; }
+template <class T>
+__device__ __forceinline__ IVT<T> operator*(T n, const IVT<T>& a) { return a * n; }
 
 // Interval x interval, 8-product form (interval.cuh:171-182).
-__device__ __forceinline__ IV operator*(const IV& a, const IV& b) {
-  double l1, l2, l3, l4, u1, u2, u3, u4;
+template <class T>
+__device__ __forceinline__ IVT<T> operator*(const IVT<T>& a, const IVT<T>& b) {
+  T l1, l2, l3, l4, u1, u2, u3, u4;
   mul_pair(a.lb, b.lb, a.lb, b.lb, l1, u1);
   mul_pair(a.lb, b.ub, a.lb, b.ub, l2, u2);
   mul_pair(a.ub, b.lb, a.ub, b.lb, l3, u3);
   mul_pair(a.ub, b.ub, a.ub, b.ub, l4, u4);
-  double ll1 = l1 < l2 ? l1 : l2, ll2 = l3 < l4 ? l3 : l4;
-  double uu1 = u1 > u2 ? u1 : u2, uu2 = u3 > u4 ? u3 : u4;
-  return IV(ll1 < ll2 ? ll1 : ll2, uu1 > uu2 ? uu1 : uu2);
+  T ll1 = l1 < l2 ? l1 : l2, ll2 = l3 < l4 ? l3 : l4;
+  T uu1 = u1 > u2 ? u1 : u2, uu2 = u3 > u4 ? u3 : u4;
+  return IVT<T>(ll1 < ll2 ? ll1 : ll2, uu1 > uu2 ? uu1 : uu2);
 }
 
 // Interval / interval with the sign cases of interval.cuh:277-316.  The six
 // cases differ only in which endpoints are divided, so the operands are
 // selected branch-free and one directed pair is computed; a divisor that
 // contains zero gives (-inf, +inf) as in the reference.
-__device__ __forceinline__ IV operator/(const IV& a, const IV& o) {
+template <class T>
+__device__ __forceinline__ IVT<T> operator/(const IVT<T>& a, const IVT<T>& o) {
   const bool pos = o.lb > 0, neg = o.ub < 0;
   const bool anneg = a.lb >= 0, anpos = a.ub <= 0;  // a >= 0, a <= 0
-  double nl, dl, nu, du;
+  T nl, dl, nu, du;
   if (pos) {
     nl = a.lb;
     dl = anneg ? o.ub : o.lb;
@@ -104,20 +118,24 @@ __device__ __forceinline__ IV operator/(const IV& a, const IV& o) {
     nu = a.lb;
     du = anneg ? o.lb : o.ub;
   }
-  if (!pos && !neg) { nl = nu = 0.0; dl = du = 1.0; }
-  IV r;
+  if (!pos && !neg) { nl = nu = T(0); dl = du = T(1); }
+  IVT<T> r;
   div_pair(nl, dl, nu, du, r.lb, r.ub);
   if (!pos && !neg) {
-    r.ub = __builtin_huge_val();
+    r.ub = (T)__builtin_huge_val();
     r.lb = -r.ub;
   }
   return r;
 }
 
-__device__ __forceinline__ UB operator<(const IV& a, const IV& b) { return UB{a.ub < b.lb, a.lb < b.ub}; }
-__device__ __forceinline__ UB operator<=(const IV& a, const IV& b) { return UB{a.ub <= b.lb, a.lb <= b.ub}; }
-__device__ __forceinline__ UB operator>(const IV& a, const IV& b) { return b < a; }
-__device__ __forceinline__ UB operator>=(const IV& a, const IV& b) { return b <= a; }
+template <class T>
+__device__ __forceinline__ UB operator<(const IVT<T>& a, const IVT<T>& b) { return UB{a.ub < b.lb, a.lb < b.ub}; }
+template <class T>
+__device__ __forceinline__ UB operator<=(const IVT<T>& a, const IVT<T>& b) { return UB{a.ub <= b.lb, a.lb <= b.ub}; }
+template <class T>
+__device__ __forceinline__ UB operator>(const IVT<T>& a, const IVT<T>& b) { return b < a; }
+template <class T>
+__device__ __forceinline__ UB operator>=(const IVT<T>& a, const IVT<T>& b) { return b <= a; }
 
 // get_subinterval (subintervals.hpp:31-42) in round-to-nearest with the
 // multiply-add fused, as nvcc compiles lb + diff*i.
@@ -127,5 +145,8 @@ __device__ __forceinline__ IV get_subinterval(IV iv, int i, int n) {
   double u = (i + 1 == n) ? iv.ub : __builtin_fma(diff, (double)(i + 1), iv.lb);
   return IV(l, u);
 }
+
+// Outward FP64 -> FP32 interval.
+__device__ __forceinline__ IVf to_f32(const IV& x) { return IVf(f32_down(x.lb), f32_up(x.ub)); }
 
 }  // namespace cpl

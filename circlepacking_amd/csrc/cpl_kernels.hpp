@@ -26,7 +26,8 @@ struct RoundStats {
   unsigned long long vdigest;
   unsigned long long certified;
   unsigned long long unresolved;
-  unsigned long long pad[3];
+  unsigned long long fp32_settled;  // items the FP32 prefilter proved
+  unsigned long long pad[2];
 };
 
 struct RefineParams {
@@ -50,6 +51,7 @@ struct RefineParams {
   uint32_t* overflow;
   const uint32_t* in_count;  // device-side item count of the input (graph mode), or null
   int32_t digest;            // 1: accumulate the per-item digests
+  uint32_t prefilter_depths; // FP32 prefilter kernels: bit d enables it at depth d
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {

@@ -4,7 +4,10 @@
 // and the three *_configuration.cu files of the reference formula for formula
 // (each function cites its source lines).  Differences in mechanism only:
 //  * trap() becomes a trap code in `trap` (first one wins; callers stop at once);
-//  * the kernel body returns a verdict byte instead of calling store_min/printf.
+//  * the kernel body returns a verdict byte instead of calling store_min/printf;
+//  * geometry and criteria are templates on the scalar type: T = double is the
+//    certifying evaluation, T = float the outward-rounded prefilter (its
+//    constants are rounded outward so every FP32 bound encloses the FP64 one).
 #pragma once
 
 #include "cpl_asin.hpp"
@@ -25,68 +28,84 @@ __device__ __forceinline__ uint8_t vbyte(int v, int nofit_flag, int detail) {
 constexpr double critical_ratio = 0.5642;   // values.hpp:28
 constexpr double lb_pi = 3.141592653589793; // values.hpp:29
 
+template <class T> struct LaneConst;
+template <> struct LaneConst<double> {
+  static constexpr double crit = critical_ratio;  // used as an upper bound (mul_ru)
+  static constexpr double lbpi = lb_pi;           // used as a lower bound (mul_rd)
+};
+template <> struct LaneConst<float> {
+  static constexpr float crit = 0x1.20ded4p-1f;   // 0.5642 rounded up
+  static constexpr float lbpi = 0x1.921fb4p+1f;   // 3.141592653589793 rounded down
+};
+
 // Interval asin (interval.cuh:617-624).
-__device__ __forceinline__ IV asin_iv(IV x, int& trap) {
-  if (x.lb < -1.0 || x.ub > 1.0) {
+template <class T>
+__device__ __forceinline__ IVT<T> asin_iv(IVT<T> x, int& trap) {
+  if (x.lb < T(-1) || x.ub > T(1)) {
     if (!trap) trap = T_ASIN;
-    return IV(0.0, 0.0);
+    return IVT<T>(T(0), T(0));
   }
-  return IV(dasin_rd(x.lb), dasin_ru(x.ub));
+  return IVT<T>(dasin_rd(x.lb), dasin_ru(x.ub));
 }
 
 // collision_angles.cu:29-31
-__device__ __forceinline__ UB is_obtuse(IV hyp, IV s1, IV s2) { return hyp.square() > s1.square() + s2.square(); }
+template <class T>
+__device__ __forceinline__ UB is_obtuse(IVT<T> hyp, IVT<T> s1, IVT<T> s2) {
+  return hyp.square() > s1.square() + s2.square();
+}
 
 // collision_angles.cu:37-117
-__device__ __attribute__((noinline)) IV compute_collision_angle(IV lc, IV sc, IV ab, int& trap) {
+template <class T>
+__device__ __attribute__((noinline)) IVT<T> compute_collision_angle(IVT<T> lc, IVT<T> sc, IVT<T> ab, int& trap) {
+  using I = IVT<T>;
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
-  IV result(0.0, -1.0);
+  I result(T(0), T(-1));
   UB obt = is_obtuse(lc, sc, ab);
-  IV lc_sq = lc.square();
-  IV sc_sq = sc.square();
-  IV add_term = (lc_sq - sc_sq) / ab;
-  add_term.tighten_lb(0.0);
+  I lc_sq = lc.square();
+  I sc_sq = sc.square();
+  I add_term = (lc_sq - sc_sq) / ab;
+  add_term.tighten_lb(T(0));
 
   if (possibly(obt)) {
-    IV lx = 0.5 * (add_term + ab);
-    IV sx = 0.5 * (add_term - ab);
-    sx.tighten_lb(0.0);
+    I lx = T(0.5) * (add_term + ab);
+    I sx = T(0.5) * (add_term - ab);
+    sx.tighten_lb(T(0));
     sx.tighten_ub(sc.ub);
     lx.tighten_ub(lc.ub);
     if (sx.empty() || lx.empty()) {
       if (definitely(obt)) { trap = T_OBT_X; return result; }
     } else {
-      IV lquot = lx / lc;
-      lquot.tighten_ub(1.0);
-      IV squot = sx / sc;
-      squot.tighten_ub(1.0);
+      I lquot = lx / lc;
+      lquot.tighten_ub(T(1));
+      I squot = sx / sc;
+      squot.tighten_ub(T(1));
       if (lquot.empty() || squot.empty()) {
         if (definitely(obt)) { trap = T_OBT_Q; return result; }
       } else {
-        IV alpha = asin_iv(lquot, trap);
+        I alpha = asin_iv(lquot, trap);
         if (trap) return result;
-        IV beta = asin_iv(squot, trap);
+        I beta = asin_iv(squot, trap);
         if (trap) return result;
         result = alpha - beta;
-        result.tighten_lb(0.0);
+        result.tighten_lb(T(0));
       }
     }
   }
 
   if (!definitely(obt)) {
-    IV lx = 0.5 * (ab + add_term);
-    IV sx = 0.5 * (ab - add_term);
-    sx.tighten_lb(0.0);
+    I lx = T(0.5) * (ab + add_term);
+    I sx = T(0.5) * (ab - add_term);
+    sx.tighten_lb(T(0));
     if (sx.empty()) {
       if (!possibly(obt)) { trap = T_ACU_X; return result; }
     } else {
-      IV alpha = asin_iv(lx / lc, trap);
+      I alpha = asin_iv(lx / lc, trap);
       if (trap) return result;
-      IV beta = asin_iv(sx / sc, trap);
+      I beta = asin_iv(sx / sc, trap);
       if (trap) return result;
-      beta.tighten_lb(0.0);
-      IV r = alpha + beta;
+      beta.tighten_lb(T(0));
+      I r = alpha + beta;
       if (result.empty()) result = r;
       else result.do_join(r);
     }
@@ -95,94 +114,105 @@ __device__ __attribute__((noinline)) IV compute_collision_angle(IV lc, IV sc, IV
 }
 
 // collision_angles.cu:119-142
-__device__ __attribute__((noinline)) IV collision_angle_inner_inner(IV la, IV ra, IV rb, int& trap) {
-  double lb, ub;
+template <class T>
+__device__ __attribute__((noinline)) IVT<T> collision_angle_inner_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  using I = IVT<T>;
+  T lb, ub;
   {
-    IV lc = la.lb + IV(ra.ub, ra.ub);
-    IV sc = la.lb + IV(rb.ub, rb.ub);
-    IV ab = IV(ra.ub, ra.ub) + rb.ub;
+    I lc = la.lb + I(ra.ub, ra.ub);
+    I sc = la.lb + I(rb.ub, rb.ub);
+    I ab = I(ra.ub, ra.ub) + rb.ub;
     ub = compute_collision_angle(lc, sc, ab, trap).ub;
-    if (trap) return IV(0, 0);
+    if (trap) return I(T(0), T(0));
   }
   {
-    IV lc = la.ub + IV(ra.lb, ra.lb);
-    IV sc = la.ub + IV(rb.lb, rb.lb);
-    IV ab = IV(ra.lb, ra.lb) + rb.lb;
-    if (ab.lb <= 0.0) lb = 0.0;
+    I lc = la.ub + I(ra.lb, ra.lb);
+    I sc = la.ub + I(rb.lb, rb.lb);
+    I ab = I(ra.lb, ra.lb) + rb.lb;
+    if (ab.lb <= T(0)) lb = T(0);
     else lb = compute_collision_angle(lc, sc, ab, trap).lb;
   }
-  return IV(lb, ub);
+  return I(lb, ub);
 }
 
 // collision_angles.cu:144-159 (outer_inner: ca = 1 - ra, cb = la + rb) and
 // :176-191 (inner_outer: ca = la + ra, cb = 1 - rb).
-__device__ __forceinline__ IV do_collision_angle(IV ca, IV cb, IV ab, int& trap) {
+template <class T>
+__device__ __forceinline__ IVT<T> do_collision_angle(IVT<T> ca, IVT<T> cb, IVT<T> ab, int& trap) {
   UB ca_longer = (cb <= ca);
   if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab, trap);
   if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab, trap);
-  IV result = compute_collision_angle(ca, cb, ab, trap);
+  IVT<T> result = compute_collision_angle(ca, cb, ab, trap);
   if (trap) return result;
   result.do_join(compute_collision_angle(cb, ca, ab, trap));
   return result;
 }
 
 // collision_angles.cu:161-174
-__device__ __attribute__((noinline)) IV collision_angle_outer_inner(IV la, IV ra, IV rb, int& trap) {
-  double lb, ub;
-  if (possibly(2.0 * (ra + rb) <= 1.0 - la)) {
-    lb = 0.0;
+template <class T>
+__device__ __attribute__((noinline)) IVT<T> collision_angle_outer_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  using I = IVT<T>;
+  T lb, ub;
+  if (possibly(T(2) * (ra + rb) <= T(1) - la)) {
+    lb = T(0);
   } else {
-    IV pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
-    lb = do_collision_angle(1.0 - pa, la + pb, pa + pb, trap).lb;
-    if (trap) return IV(0, 0);
+    I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
+    lb = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).lb;
+    if (trap) return I(T(0), T(0));
   }
-  IV pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
-  ub = do_collision_angle(1.0 - pa, la + pb, pa + pb, trap).ub;
-  return IV(lb, ub);
+  I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
+  ub = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).ub;
+  return I(lb, ub);
 }
 
 // collision_angles.cu:193-205
-__device__ __attribute__((noinline)) IV collision_angle_inner_outer(IV la, IV ra, IV rb, int& trap) {
-  double lb, ub;
-  if (possibly(2.0 * (ra + rb) <= 1.0 - la)) {
-    lb = 0.0;
+template <class T>
+__device__ __attribute__((noinline)) IVT<T> collision_angle_inner_outer(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  using I = IVT<T>;
+  T lb, ub;
+  if (possibly(T(2) * (ra + rb) <= T(1) - la)) {
+    lb = T(0);
   } else {
-    IV pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
-    lb = do_collision_angle(la + pa, 1.0 - pb, pa + pb, trap).lb;
-    if (trap) return IV(0, 0);
+    I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
+    lb = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).lb;
+    if (trap) return I(T(0), T(0));
   }
-  IV pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
-  ub = do_collision_angle(la + pa, 1.0 - pb, pa + pb, trap).ub;
-  return IV(lb, ub);
+  I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
+  ub = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).ub;
+  return I(lb, ub);
 }
 
 // collision_angles.cu:207-231
-__device__ __attribute__((noinline)) IV collision_angle_outer_outer(IV ra, IV rb, int& trap) {
-  double lb, ub;
+template <class T>
+__device__ __attribute__((noinline)) IVT<T> collision_angle_outer_outer(IVT<T> ra, IVT<T> rb, int& trap) {
+  using I = IVT<T>;
+  T lb, ub;
   {
-    IV lc = IV(1.0, 1.0) - rb.ub;
-    IV sc = IV(1.0, 1.0) - ra.ub;
-    IV ab = IV(ra.ub, ra.ub) + rb.ub;
+    I lc = I(T(1), T(1)) - rb.ub;
+    I sc = I(T(1), T(1)) - ra.ub;
+    I ab = I(ra.ub, ra.ub) + rb.ub;
     ub = compute_collision_angle(lc, sc, ab, trap).ub;
-    if (trap) return IV(0, 0);
+    if (trap) return I(T(0), T(0));
   }
   {
-    IV lc = IV(1.0, 1.0) - rb.lb;
-    IV sc = IV(1.0, 1.0) - ra.lb;
-    IV ab = IV(ra.lb, ra.lb) + rb.lb;
-    if (ab.lb <= 0.0) lb = 0.0;
+    I lc = I(T(1), T(1)) - rb.lb;
+    I sc = I(T(1), T(1)) - ra.lb;
+    I ab = I(ra.lb, ra.lb) + rb.lb;
+    if (ab.lb <= T(0)) lb = T(0);
     else lb = compute_collision_angle(lc, sc, ab, trap).lb;
   }
-  return IV(lb, ub);
+  return I(lb, ub);
 }
 
 // collision_angles.cu:233-238
-__device__ __forceinline__ IV semicircle_angle_outer(IV r) {
-  return IV(dasin_rd(div_rd(r.lb, add_ru(1.0, -r.lb))), dasin_ru(div_ru(r.ub, add_rd(1.0, -r.ub))));
+template <class T>
+__device__ __forceinline__ IVT<T> semicircle_angle_outer(IVT<T> r) {
+  return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(T(1), -r.lb))), dasin_ru(div_ru(r.ub, add_rd(T(1), -r.ub))));
 }
 // collision_angles.cu:240-246
-__device__ __forceinline__ IV semicircle_angle_inner(IV la, IV r) {
-  return IV(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))), dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
+template <class T>
+__device__ __forceinline__ IVT<T> semicircle_angle_inner(IVT<T> la, IVT<T> r) {
+  return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))), dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
 }
 
 // lower_bound_radius_single_placement.cu:26-61
@@ -203,7 +233,7 @@ __device__ __attribute__((noinline)) double lower_bound_radius_single_placement(
   }
 }
 
-// ---- admissible ranges (a11) ------------------------------------------------
+// ---- admissible ranges (a11), FP64 only ------------------------------------
 // start_configuration.cu:58-66 = middle_configuration.cu:66-74 = end_configuration.cu:55-63
 __device__ __forceinline__ IV compute_r1_range(IV la, double lbrsp) {
   double ub_by_lane_width = (0.5 * (1.0 - la)).ub;
@@ -230,177 +260,249 @@ __device__ __forceinline__ IV r3_range(IV la, IV r2) {
 }
 
 // ---- criteria ------------------------------------------------------------
-__device__ __forceinline__ double ub_ratio_of(IV la) { return add_ru(1.0, -mul_rd(la.lb, la.lb)); }
-__device__ __forceinline__ double lb_disk(IV r) { return mul_rd(lb_pi, mul_rd(r.lb, r.lb)); }
+template <class T>
+__device__ __forceinline__ T ub_ratio_of(IVT<T> la) { return add_ru(T(1), -mul_rd(la.lb, la.lb)); }
+template <class T>
+__device__ __forceinline__ T lb_disk(IVT<T> r) { return mul_rd(LaneConst<T>::lbpi, mul_rd(r.lb, r.lb)); }
+template <class T>
+__device__ __forceinline__ T crit_of(T x) { return mul_ru(LaneConst<T>::crit, x); }
 
 // start_configuration.cu:93-101
-__device__ __forceinline__ bool handle_case_1(IV la, IV r1, IV r2, IV a12) {
-  double ub_ratio = ub_ratio_of(la);
-  double lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2);
-  IV a = semicircle_angle_outer(r1) + a12;
-  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
-  double lb_disk_area = add_rd(lb_area1, mul_rd(0.5, lb_area2));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+template <class T>
+__device__ __forceinline__ bool handle_case_1(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> a12) {
+  T ub_ratio = ub_ratio_of(la);
+  T lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2);
+  IVT<T> a = semicircle_angle_outer(r1) + a12;
+  T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), a.ub));
+  T lb_disk_area = add_rd(lb_area1, mul_rd(T(0.5), lb_area2));
+  return lb_disk_area >= crit_of(ub_area_used);
 }
 // start_configuration.cu:103-112
-__device__ __forceinline__ bool handle_case_2(IV la, IV r1, IV r2, IV r3, IV a13) {
-  double ub_ratio = ub_ratio_of(la);
-  double lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2), lb_area3 = lb_disk(r3);
-  IV a = semicircle_angle_outer(r1) + a13;
-  double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
-  double lb_disk_area = add_rd(add_rd(lb_area1, lb_area2), mul_rd(0.5, lb_area3));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+template <class T>
+__device__ __forceinline__ bool handle_case_2(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, IVT<T> a13) {
+  T ub_ratio = ub_ratio_of(la);
+  T lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2), lb_area3 = lb_disk(r3);
+  IVT<T> a = semicircle_angle_outer(r1) + a13;
+  T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), a.ub));
+  T lb_disk_area = add_rd(add_rd(lb_area1, lb_area2), mul_rd(T(0.5), lb_area3));
+  return lb_disk_area >= crit_of(ub_area_used);
 }
 // start_configuration.cu:169-188
-__device__ __forceinline__ bool handle_2d_case(IV la, IV r1, IV r2, int& trap) {
-  IV a1 = semicircle_angle_outer(r1);
-  IV a2 = semicircle_angle_inner(la, r2);
-  IV a12 = collision_angle_outer_inner(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_2d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap) {
+  IVT<T> a1 = semicircle_angle_outer(r1);
+  IVT<T> a2 = semicircle_angle_inner(la, r2);
+  IVT<T> a12 = collision_angle_outer_inner(la, r1, r2, trap);
   if (trap) return false;
-  IV total_angle = a1 + a2 + a12;
-  IV reuse_angle = a12 + a2 - a1;
-  reuse_angle.tighten_lb(0.0);
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, total_angle.ub));
-  IV rho = 1.0 - (la + 2.0 * r2).square();
-  rho.tighten_lb(0.0);
-  double lb_area_reusable = mul_rd(mul_rd(0.5, reuse_angle.lb), rho.lb);
-  double lb_disk_area = add_rd(lb_disk(r2), lb_disk(r1));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  IVT<T> total_angle = a1 + a2 + a12;
+  IVT<T> reuse_angle = a12 + a2 - a1;
+  reuse_angle.tighten_lb(T(0));
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), total_angle.ub));
+  IVT<T> rho = T(1) - (la + T(2) * r2).square();
+  rho.tighten_lb(T(0));
+  T lb_area_reusable = mul_rd(mul_rd(T(0.5), reuse_angle.lb), rho.lb);
+  T lb_disk_area = add_rd(lb_disk(r2), lb_disk(r1));
+  return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
 }
 // start_configuration.cu:227-265 (outer_outer(r1, r2) at :232 kept as in the reference)
-__device__ __forceinline__ bool handle_3d_case(IV la, IV r1, IV r2, IV r3, int& trap) {
-  IV a1 = semicircle_angle_outer(r1);
-  IV a2 = semicircle_angle_inner(la, r2);
-  IV a3 = semicircle_angle_outer(r3);
-  IV a12 = collision_angle_outer_inner(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_3d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+  using I = IVT<T>;
+  I a1 = semicircle_angle_outer(r1);
+  I a2 = semicircle_angle_inner(la, r2);
+  I a3 = semicircle_angle_outer(r3);
+  I a12 = collision_angle_outer_inner(la, r1, r2, trap);
   if (trap) return false;
-  IV a13 = collision_angle_outer_outer(r1, r2, trap);
+  I a13 = collision_angle_outer_outer(r1, r2, trap);
   if (trap) return false;
-  IV a23 = collision_angle_inner_outer(la, r2, r3, trap);
+  I a23 = collision_angle_inner_outer(la, r2, r3, trap);
   if (trap) return false;
-  IV b13_123 = a12 + a23;
-  IV b13(a13.lb < b13_123.lb ? b13_123.lb : a13.lb, a13.ub < b13_123.ub ? b13_123.ub : a13.ub);
-  IV a123 = a1 + b13 + a3;
-  IV a122 = a1 + a12 + a2;
-  IV a(a123.lb < a122.lb ? a122.lb : a123.lb, a123.ub < a122.ub ? a122.ub : a123.ub);
-  double lb_reusable_angle = (a - a122).lb;
-  if (lb_reusable_angle < 0.0) lb_reusable_angle = 0.0;
-  double lb_outer_rad = add_rd(1.0, -mul_ru(2.0, r3.ub));
-  double lb_outer_rad_sq = mul_rd(lb_outer_rad, lb_outer_rad);
-  double ub_inner_rad_sq = mul_ru(la.ub, la.ub);
-  double lb_area_reused = mul_rd(mul_rd(0.5, lb_reusable_angle), add_rd(lb_outer_rad_sq, -ub_inner_rad_sq));
-  if (lb_area_reused < 0.0) lb_area_reused = 0.0;
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, a.ub));
+  I b13_123 = a12 + a23;
+  I b13(a13.lb < b13_123.lb ? b13_123.lb : a13.lb, a13.ub < b13_123.ub ? b13_123.ub : a13.ub);
+  I a123 = a1 + b13 + a3;
+  I a122 = a1 + a12 + a2;
+  I a(a123.lb < a122.lb ? a122.lb : a123.lb, a123.ub < a122.ub ? a122.ub : a123.ub);
+  T lb_reusable_angle = (a - a122).lb;
+  if (lb_reusable_angle < T(0)) lb_reusable_angle = T(0);
+  T lb_outer_rad = add_rd(T(1), -mul_ru(T(2), r3.ub));
+  T lb_outer_rad_sq = mul_rd(lb_outer_rad, lb_outer_rad);
+  T ub_inner_rad_sq = mul_ru(la.ub, la.ub);
+  T lb_area_reused = mul_rd(mul_rd(T(0.5), lb_reusable_angle), add_rd(lb_outer_rad_sq, -ub_inner_rad_sq));
+  if (lb_area_reused < T(0)) lb_area_reused = T(0);
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
   ub_area_used = add_ru(ub_area_used, -lb_area_reused);
-  double lb_disk_area = add_rd(lb_disk(r1), add_rd(lb_disk(r2), lb_disk(r3)));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+  T lb_disk_area = add_rd(lb_disk(r1), add_rd(lb_disk(r2), lb_disk(r3)));
+  return lb_disk_area >= crit_of(ub_area_used);
 }
 // middle_configuration.cu:86-154 (the two cases share this tail)
-__device__ __forceinline__ bool middle_area_checks(IV la, IV r1, IV r2, IV r3, IV c12, IV c13, IV c23) {
-  double ub_ratio = ub_ratio_of(la);
+template <class T>
+__device__ __forceinline__ bool middle_area_checks(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, IVT<T> c12, IVT<T> c13,
+                                                   IVT<T> c23) {
+  T ub_ratio = ub_ratio_of(la);
   UB case_a = (c12 + c23 >= c13);
-  double lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2);
+  T lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2);
   if (possibly(case_a)) {
-    double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c12.ub));
-    double lb_disk_area = mul_rd(0.5, add_rd(lb_area1, lb_area2));
-    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+    T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), c12.ub));
+    T lb_disk_area = mul_rd(T(0.5), add_rd(lb_area1, lb_area2));
+    if (lb_disk_area < crit_of(ub_area_used)) return false;
   }
   if (possibly(!case_a)) {
-    double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c13.ub));
-    double lb_area3 = lb_disk(r3);
-    double lb_disk_area = add_rd(mul_rd(0.5, add_rd(lb_area1, lb_area3)), lb_area2);
-    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+    T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), c13.ub));
+    T lb_area3 = lb_disk(r3);
+    T lb_disk_area = add_rd(mul_rd(T(0.5), add_rd(lb_area1, lb_area3)), lb_area2);
+    if (lb_disk_area < crit_of(ub_area_used)) return false;
   }
   return true;
 }
-__device__ __forceinline__ bool handle_case_1_1(IV la, IV r1, IV r2, IV r3, int& trap) {
-  IV c12 = collision_angle_inner_outer(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_case_1_1(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+  IVT<T> c12 = collision_angle_inner_outer(la, r1, r2, trap);
   if (trap) return false;
-  IV c13 = collision_angle_inner_inner(la, r1, r3, trap);
+  IVT<T> c13 = collision_angle_inner_inner(la, r1, r3, trap);
   if (trap) return false;
-  IV c23 = collision_angle_outer_inner(la, r2, r3, trap);
+  IVT<T> c23 = collision_angle_outer_inner(la, r2, r3, trap);
   if (trap) return false;
   return middle_area_checks(la, r1, r2, r3, c12, c13, c23);
 }
-__device__ __forceinline__ bool handle_case_2_1(IV la, IV r1, IV r2, IV r3, int& trap) {
-  IV c12 = collision_angle_outer_inner(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_case_2_1(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+  IVT<T> c12 = collision_angle_outer_inner(la, r1, r2, trap);
   if (trap) return false;
-  IV c13 = collision_angle_outer_outer(r1, r3, trap);
+  IVT<T> c13 = collision_angle_outer_outer(r1, r3, trap);
   if (trap) return false;
-  IV c23 = collision_angle_inner_outer(la, r2, r3, trap);
+  IVT<T> c23 = collision_angle_inner_outer(la, r2, r3, trap);
   if (trap) return false;
   return middle_area_checks(la, r1, r2, r3, c12, c13, c23);
 }
 // end_configuration.cu:71-85
-__device__ __forceinline__ bool handle_inner_outer_2d(IV la, IV r1, IV r2, int& trap) {
-  IV a = collision_angle_inner_outer(la, r1, r2, trap) + semicircle_angle_outer(r2);
+template <class T>
+__device__ __forceinline__ bool handle_inner_outer_2d(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap) {
+  IVT<T> a = collision_angle_inner_outer(la, r1, r2, trap) + semicircle_angle_outer(r2);
   if (trap) return false;
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, a.ub));
-  IV rho = (1.0 - 2.0 * r2).square() - la.square();
-  rho.tighten_lb(0.0);
-  double lb_alpha_reusable = (a - semicircle_angle_inner(la, r1)).lb;
-  double lb_area_reusable = mul_rd(mul_rd(0.5, lb_alpha_reusable), rho.lb);
-  double lb_disk_area = add_rd(lb_disk(r2), mul_rd(0.5, lb_disk(r1)));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
+  IVT<T> rho = (T(1) - T(2) * r2).square() - la.square();
+  rho.tighten_lb(T(0));
+  T lb_alpha_reusable = (a - semicircle_angle_inner(la, r1)).lb;
+  T lb_area_reusable = mul_rd(mul_rd(T(0.5), lb_alpha_reusable), rho.lb);
+  T lb_disk_area = add_rd(lb_disk(r2), mul_rd(T(0.5), lb_disk(r1)));
+  return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
 }
 // end_configuration.cu:87-101
-__device__ __forceinline__ bool handle_outer_inner_2d(IV la, IV r1, IV r2, int& trap) {
-  IV a = collision_angle_outer_inner(la, r1, r2, trap) + semicircle_angle_inner(la, r2);
+template <class T>
+__device__ __forceinline__ bool handle_outer_inner_2d(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap) {
+  IVT<T> a = collision_angle_outer_inner(la, r1, r2, trap) + semicircle_angle_inner(la, r2);
   if (trap) return false;
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, a.ub));
-  IV rho = 1.0 - (la + 2.0 * r2).square();
-  rho.tighten_lb(0.0);
-  double lb_alpha_reusable = (a - semicircle_angle_outer(r1)).lb;
-  double lb_area_reusable = mul_rd(mul_rd(0.5, lb_alpha_reusable), rho.lb);
-  double lb_disk_area = add_rd(lb_disk(r2), mul_rd(0.5, lb_disk(r1)));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
+  IVT<T> rho = T(1) - (la + T(2) * r2).square();
+  rho.tighten_lb(T(0));
+  T lb_alpha_reusable = (a - semicircle_angle_outer(r1)).lb;
+  T lb_area_reusable = mul_rd(mul_rd(T(0.5), lb_alpha_reusable), rho.lb);
+  T lb_disk_area = add_rd(lb_disk(r2), mul_rd(T(0.5), lb_disk(r1)));
+  return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
 }
 // end_configuration.cu:154-181
-__device__ __forceinline__ bool handle_r1_inner_3d(IV la, IV r1, IV r2, IV r3, int& trap) {
-  IV a12 = collision_angle_inner_outer(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_r1_inner_3d(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+  using I = IVT<T>;
+  I a12 = collision_angle_inner_outer(la, r1, r2, trap);
   if (trap) return false;
-  IV a23 = collision_angle_outer_inner(la, r2, r3, trap);
+  I a23 = collision_angle_outer_inner(la, r2, r3, trap);
   if (trap) return false;
-  IV a13 = collision_angle_inner_inner(la, r1, r3, trap);
+  I a13 = collision_angle_inner_inner(la, r1, r3, trap);
   if (trap) return false;
   if (definitely(a12 + a23 >= a13)) return true;
-  IV a3 = semicircle_angle_inner(la, r3);
-  IV a = a13 + a3;
-  IV a2 = semicircle_angle_outer(r2);
-  IV reusable_alpha = a - (a12 + a2);
-  IV rho = 1.0 - (la + 2.0 * r3).square();
-  rho.tighten_lb(0.0);
-  double lb_area_reusable = mul_rd(mul_rd(0.5, reusable_alpha.lb), rho.lb);
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, a.ub));
-  double lb_disk_area = add_rd(add_rd(lb_disk(r2), lb_disk(r3)), mul_rd(0.5, lb_disk(r1)));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  I a3 = semicircle_angle_inner(la, r3);
+  I a = a13 + a3;
+  I a2 = semicircle_angle_outer(r2);
+  I reusable_alpha = a - (a12 + a2);
+  I rho = T(1) - (la + T(2) * r3).square();
+  rho.tighten_lb(T(0));
+  T lb_area_reusable = mul_rd(mul_rd(T(0.5), reusable_alpha.lb), rho.lb);
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
+  T lb_disk_area = add_rd(add_rd(lb_disk(r2), lb_disk(r3)), mul_rd(T(0.5), lb_disk(r1)));
+  return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
 }
 // end_configuration.cu:183-210
-__device__ __forceinline__ bool handle_r1_outer_3d(IV la, IV r1, IV r2, IV r3, int& trap) {
-  IV a12 = collision_angle_outer_inner(la, r1, r2, trap);
+template <class T>
+__device__ __forceinline__ bool handle_r1_outer_3d(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+  using I = IVT<T>;
+  I a12 = collision_angle_outer_inner(la, r1, r2, trap);
   if (trap) return false;
-  IV a23 = collision_angle_inner_outer(la, r2, r3, trap);
+  I a23 = collision_angle_inner_outer(la, r2, r3, trap);
   if (trap) return false;
-  IV a13 = collision_angle_outer_outer(r1, r3, trap);
+  I a13 = collision_angle_outer_outer(r1, r3, trap);
   if (trap) return false;
   if (definitely(a12 + a23 >= a13)) return true;
-  IV a3 = semicircle_angle_outer(r3);
-  IV a = a13 + a3;
-  IV a2 = semicircle_angle_inner(la, r2);
-  IV reusable_alpha = a - (a12 + a2);
-  IV rho = (1.0 - 2.0 * r3).square() - la.square();
-  rho.tighten_lb(0.0);
-  double lb_area_reusable = mul_rd(mul_rd(0.5, reusable_alpha.lb), rho.lb);
-  double ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(0.5, a.ub));
-  double lb_disk_area = add_rd(add_rd(lb_disk(r2), lb_disk(r3)), mul_rd(0.5, lb_disk(r1)));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  I a3 = semicircle_angle_outer(r3);
+  I a = a13 + a3;
+  I a2 = semicircle_angle_inner(la, r2);
+  I reusable_alpha = a - (a12 + a2);
+  I rho = (T(1) - T(2) * r3).square() - la.square();
+  rho.tighten_lb(T(0));
+  T lb_area_reusable = mul_rd(mul_rd(T(0.5), reusable_alpha.lb), rho.lb);
+  T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
+  T lb_disk_area = add_rd(add_rd(lb_disk(r2), lb_disk(r3)), mul_rd(T(0.5), lb_disk(r1)));
+  return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
+}
+
+// The gated criteria of family F on tightened cells.  Returns 0 (all hold),
+// the 1-based failing criterion, or 16 + trap code.
+template <int F, class T>
+__device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3) {
+  int trap = 0;
+  if constexpr (F == F_START) {  // start_configuration.cu:142-162
+    IVT<T> a12 = collision_angle_outer_inner(la, r1, r2, trap);
+    if (trap) return 16 + trap;
+    IVT<T> a13 = collision_angle_outer_outer(r1, r3, trap);
+    if (trap) return 16 + trap;
+    IVT<T> a23 = collision_angle_inner_outer(la, r2, r3, trap);
+    if (trap) return 16 + trap;
+    UB case1 = (a12 + a23 > a13);
+    if (possibly(case1) && !handle_case_1(la, r1, r2, a12)) return 1;
+    if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13)) return 2;
+    return 0;
+  } else if constexpr (F == F_START_END_2D) {
+    bool ok = handle_2d_case(la, r1, r2, trap);
+    if (trap) return 16 + trap;
+    return ok ? 0 : 1;
+  } else if constexpr (F == F_START_END_3D) {
+    bool ok = handle_3d_case(la, r1, r2, r3, trap);
+    if (trap) return 16 + trap;
+    return ok ? 0 : 1;
+  } else if constexpr (F == F_MIDDLE) {
+    bool ok = handle_case_1_1(la, r1, r2, r3, trap);
+    if (trap) return 16 + trap;
+    if (!ok) return 1;
+    ok = handle_case_2_1(la, r1, r2, r3, trap);
+    if (trap) return 16 + trap;
+    return ok ? 0 : 2;
+  } else if constexpr (F == F_END_2D) {
+    bool ok = handle_inner_outer_2d(la, r1, r2, trap);
+    if (trap) return 16 + trap;
+    if (!ok) return 1;
+    ok = handle_outer_inner_2d(la, r1, r2, trap);
+    if (trap) return 16 + trap;
+    return ok ? 0 : 2;
+  } else {
+    bool ok = handle_r1_inner_3d(la, r1, r2, r3, trap);
+    if (trap) return 16 + trap;
+    if (!ok) return 1;
+    ok = handle_r1_outer_3d(la, r1, r2, r3, trap);
+    if (trap) return 16 + trap;
+    return ok ? 0 : 2;
+  }
 }
 
 // ---- kernel body (tighten, empty, fit check, gated criteria) ----------------
 // start_configuration.cu:127-162, :200-212, :280-296; middle_configuration.cu:168-202;
 // end_configuration.cu:110-143, :226-265.
-template <int F>
-__device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3) {
+// PREFILTER (and `pre` at run time): the criteria are first evaluated on the
+// FP32 outward enclosure of the tightened cell; a cell FP32 proves is PROVEN,
+// every other cell is decided by the FP64 evaluation.  `*fp32_hit` is set when
+// FP32 settled the cell.
+template <int F, bool PREFILTER = false>
+__device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3, bool pre = true,
+                                                    bool* fp32_hit = nullptr) {
   IV r2 = r2_;
   if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
   IV r1 = r1_;
@@ -413,50 +515,19 @@ __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3
       nofit = 1;
     }
   }
-  int trap = 0;
-  uint8_t v;
-  if constexpr (F == F_START) {
-    IV a12 = collision_angle_outer_inner(la, r1, r2, trap);
-    if (trap) return vbyte(V_TRAP, nofit, trap);
-    IV a13 = collision_angle_outer_outer(r1, r3, trap);
-    if (trap) return vbyte(V_TRAP, nofit, trap);
-    IV a23 = collision_angle_inner_outer(la, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, nofit, trap);
-    UB case1 = (a12 + a23 > a13);
-    if (possibly(case1) && !handle_case_1(la, r1, r2, a12)) return vbyte(V_CRITICAL, nofit, 1);
-    if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13)) return vbyte(V_CRITICAL, nofit, 2);
-    v = vbyte(V_PROVEN, nofit, 0);
-  } else if constexpr (F == F_START_END_2D) {
-    bool ok = handle_2d_case(la, r1, r2, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    v = ok ? vbyte(V_PROVEN, 0, 0) : vbyte(V_CRITICAL, 0, 1);
-  } else if constexpr (F == F_START_END_3D) {
-    bool ok = handle_3d_case(la, r1, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    v = ok ? vbyte(V_PROVEN, 0, 0) : vbyte(V_CRITICAL, 0, 1);
-  } else if constexpr (F == F_MIDDLE) {
-    bool ok = handle_case_1_1(la, r1, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    if (!ok) return vbyte(V_CRITICAL, 0, 1);
-    ok = handle_case_2_1(la, r1, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    v = ok ? vbyte(V_PROVEN, 0, 0) : vbyte(V_CRITICAL, 0, 2);
-  } else if constexpr (F == F_END_2D) {
-    bool ok = handle_inner_outer_2d(la, r1, r2, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    if (!ok) return vbyte(V_CRITICAL, 0, 1);
-    ok = handle_outer_inner_2d(la, r1, r2, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    v = ok ? vbyte(V_PROVEN, 0, 0) : vbyte(V_CRITICAL, 0, 2);
-  } else {
-    bool ok = handle_r1_inner_3d(la, r1, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    if (!ok) return vbyte(V_CRITICAL, 0, 1);
-    ok = handle_r1_outer_3d(la, r1, r2, r3, trap);
-    if (trap) return vbyte(V_TRAP, 0, trap);
-    v = ok ? vbyte(V_PROVEN, 0, 0) : vbyte(V_CRITICAL, 0, 2);
+  if constexpr (PREFILTER) {
+    if (pre) {
+      int c32 = criteria<F, float>(to_f32(la), to_f32(r1), to_f32(r2), to_f32(r3));
+      if (c32 == 0) {
+        if (fp32_hit) *fp32_hit = true;
+        return vbyte(V_PROVEN, nofit, 0);
+      }
+    }
   }
-  return v;
+  int c = criteria<F, double>(la, r1, r2, r3);
+  if (c >= 16) return vbyte(V_TRAP, nofit, c - 16);
+  if (c) return vbyte(V_CRITICAL, nofit, c);
+  return vbyte(V_PROVEN, nofit, 0);
 }
 
 // r3 from a position interval p in [0,1] over the admissible r3 range; for a
@@ -469,8 +540,9 @@ __device__ __forceinline__ IV r3_from_position(IV range, IV p) {
 }
 
 // Synthetic cell (la, r1, r2, p3) of family F; lbrsp = lower_bound_radius_single_placement(la.lb).
-template <int F>
-__device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp) {
+template <int F, bool PREFILTER = false>
+__device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp,
+                                                  bool pre = true, bool* fp32_hit = nullptr) {
   IV r1c = r1, r2c = r2;
   if (clip) {
     r1c = r1.intersect(compute_r1_range(la, lbrsp));
@@ -484,7 +556,7 @@ __device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bo
     if (range.empty()) return vbyte(V_OOD, 0, 0);
     r3 = r3_from_position(range, p3);
   }
-  return eval_kernel_body<F>(la, r1c, r2c, r3);
+  return eval_kernel_body<F, PREFILTER>(la, r1c, r2c, r3, pre, fp32_hit);
 }
 
 }  // namespace cpl

@@ -219,3 +219,92 @@ __device__ __forceinline__ double sqrt_rd(double x) { return sqrt_dir<false>(x);
 __device__ __forceinline__ double sqrt_ru(double x) { return sqrt_dir<true>(x); }
 
 }  // namespace cpl
+
+// ============================================================================
+// FP32 primitives for the outward-rounded prefilter.  MODE[1:0] is the FP32
+// rounding field.  The prefilter only needs enclosures (its results are never
+// reported), so operands outside the comfortable range widen to +-inf instead
+// of taking an exact slow path.
+// ============================================================================
+namespace cpl {
+
+#define CPL_MODE32_RD "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 2\n\t"
+#define CPL_MODE32_RU "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 1\n\t"
+#define CPL_MODE32_RN "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 0\n\t"
+
+#define CPL_DIR_BINOP32(NAME, MODE, INSN)                                                    \
+  __device__ __forceinline__ float NAME(float a, float b) {                                 \
+    float r;                                                                                \
+    asm("s_nop 1\n\t" MODE INSN " %0, %1, %2\n\ts_nop 0\n\t" CPL_MODE32_RN                  \
+        : "=v"(r)                                                                           \
+        : "v"(a), "v"(b));                                                                  \
+    return r;                                                                               \
+  }
+CPL_DIR_BINOP32(add_rd, CPL_MODE32_RD, "v_add_f32")
+CPL_DIR_BINOP32(add_ru, CPL_MODE32_RU, "v_add_f32")
+CPL_DIR_BINOP32(mul_rd, CPL_MODE32_RD, "v_mul_f32")
+CPL_DIR_BINOP32(mul_ru, CPL_MODE32_RU, "v_mul_f32")
+#undef CPL_DIR_BINOP32
+
+__device__ __forceinline__ void add_pair(float al, float bl, float au, float bu, float& l, float& u) {
+  asm("s_nop 1\n\t" CPL_MODE32_RD "v_add_f32 %0, %2, %3\n\ts_nop 0\n\t" CPL_MODE32_RU
+      "v_add_f32 %1, %4, %5\n\ts_nop 0\n\t" CPL_MODE32_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bl), "v"(au), "v"(bu));
+}
+__device__ __forceinline__ void sub_pair(float al, float bu, float au, float bl, float& l, float& u) {
+  asm("s_nop 1\n\t" CPL_MODE32_RD "v_sub_f32 %0, %2, %3\n\ts_nop 0\n\t" CPL_MODE32_RU
+      "v_sub_f32 %1, %4, %5\n\ts_nop 0\n\t" CPL_MODE32_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bu), "v"(au), "v"(bl));
+}
+__device__ __forceinline__ void mul_pair(float al, float bl, float au, float bu, float& l, float& u) {
+  asm("s_nop 1\n\t" CPL_MODE32_RD "v_mul_f32 %0, %2, %3\n\ts_nop 0\n\t" CPL_MODE32_RU
+      "v_mul_f32 %1, %4, %5\n\ts_nop 0\n\t" CPL_MODE32_RN
+      : "=&v"(l), "=v"(u)
+      : "v"(al), "v"(bl), "v"(au), "v"(bu));
+}
+
+// Division: q = RN(a/b) (IEEE), then fma(-q, b, a) and fma(r, y, q) in the
+// target mode with y = +-2^-80: the result is on the correct side of the true
+// quotient even when q is only faithful.  Outside |q|, |b| in [2^-50, 2^50]
+// the bound widens to -inf / +inf.
+__device__ __forceinline__ bool div_fast_ok32(float q, float b) {
+  float aq = __builtin_fabsf(q), ab = __builtin_fabsf(b);
+  return (aq >= 0x1p-50f) & (aq < 0x1p+50f) & (ab >= 0x1p-50f) & (ab < 0x1p+50f);
+}
+__device__ __forceinline__ void div_pair(float al, float bl, float au, float bu, float& l, float& u) {
+  float ql = al / bl, qu = au / bu;
+  float yl = __builtin_copysignf(0x1p-80f, bl), yu = __builtin_copysignf(0x1p-80f, bu), rl, ru;
+  asm("s_nop 1\n\t" CPL_MODE32_RD "v_fma_f32 %2, -%4, %5, %6\n\t"
+      "v_fma_f32 %0, %2, %7, %4\n\t" CPL_MODE32_RU "v_fma_f32 %3, -%8, %9, %10\n\t"
+      "v_fma_f32 %1, %3, %11, %8\n\t" CPL_MODE32_RN
+      : "=&v"(l), "=&v"(u), "=&v"(rl), "=&v"(ru)
+      : "v"(ql), "v"(bl), "v"(al), "v"(yl), "v"(qu), "v"(bu), "v"(au), "v"(yu));
+  if (!div_fast_ok32(ql, bl)) l = (al == 0.0f && bl != 0.0f) ? ql : -__builtin_huge_valf();
+  if (!div_fast_ok32(qu, bu)) u = (au == 0.0f && bu != 0.0f) ? qu : __builtin_huge_valf();
+}
+__device__ __forceinline__ float div_rd(float a, float b) {
+  float l, u;
+  div_pair(a, b, a, b, l, u);
+  return l;
+}
+__device__ __forceinline__ float div_ru(float a, float b) {
+  float l, u;
+  div_pair(a, b, a, b, l, u);
+  return u;
+}
+
+// Outward conversion of a double to float.
+__device__ __forceinline__ float next_down_f(float f) {
+  if (f != f || f == -__builtin_huge_valf()) return f;
+  if (f == 0.0f) return -0x1p-149f;
+  return __int_as_float(__float_as_int(f) + (f > 0.0f ? -1 : 1));
+}
+__device__ __forceinline__ float f32_down(double x) {
+  float f = (float)x;  // round to nearest
+  return ((double)f > x) ? next_down_f(f) : f;
+}
+__device__ __forceinline__ float f32_up(double x) { return -f32_down(-x); }
+
+}  // namespace cpl

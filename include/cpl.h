@@ -96,7 +96,8 @@ typedef struct {
   uint32_t options;           /* CPL_OPT_* (0: the plain FP64 path, one launch per family and round) */
   int32_t block;              /* threads per block of the evaluation kernels (0: 256) */
   int32_t cells_per_thread;   /* items a thread takes per scheduling step (0: 1) */
-  int32_t reserved[4];
+  uint32_t prefilter_depths;  /* with CPL_OPT_FP32: bit d enables the prefilter at depth d (0: every depth) */
+  int32_t reserved[3];
 } cpl_refine_cfg;
 
 /* Execution options of cpl_refine_run.  Every option changes scheduling or
@@ -137,6 +138,7 @@ typedef struct {
   double seconds;                                  /* device time of all rounds (HIP events) */
   double eval_seconds;                             /* device time of the criteria kernels only */
   uint64_t peak_items;                             /* largest worklist held on the device */
+  uint64_t fp32_settled[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES]; /* items the FP32 prefilter proved (CPL_OPT_FP32) */
 } cpl_refine_stats;
 
 /* ---- lifecycle --------------------------------------------------------- */
@@ -195,6 +197,24 @@ int cpl_autotune(cpl_refine_cfg* cfg, int reps, cpl_tune_point* points, int* n_p
  * 11 dasin_rd 12 dasin_ru 13 lower_bound_radius_single_placement(a)
  * 14 div_rn 15 sqrt_rn. */
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n);
+/* FP32 prefilter primitives through the same entry point (operands and
+ * results are float values carried in doubles): 16 add_rd 17 add_ru 18 sub_rd
+ * 19 sub_ru 20 mul_rd 21 mul_ru 22 div_rd 23 div_ru 24 asin_rn 25 dasin_rd
+ * 26 dasin_ru; 27 / 28: a double rounded down / up to float. */
+
+/* Geometry of the criteria on n records of 6 doubles (la, ra, rb intervals):
+ * fn 0 collision_angle_inner_inner 1 outer_inner 2 inner_outer (collision_angles.cu:119-205)
+ * 3 outer_outer(ra, rb) (:207-231) 4 semicircle_angle_outer(ra) (:233-238)
+ * 5 semicircle_angle_inner(la, ra) (:240-246) 6 compute_collision_angle(la, ra, rb) (:37-117).
+ * fp32 = 1 evaluates in the prefilter precision on the outward FP32 enclosure
+ * of the inputs.  out: n x (lb, ub); trap: n trap codes (0: none). */
+int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, size_t n);
+
+/* For n cells (8 doubles each, as cpl_eval_kernel_body): the FP32 criteria
+ * result on the tightened cell (0: every criterion holds, k: criterion k fails
+ * first, 16 + trap code; 255 when the cell is decided before the criteria)
+ * and the FP64 verdict byte. */
+int cpl_debug_prefilter(int family, const double* cells, size_t n, uint8_t* fp32_result, uint8_t* verdicts);
 
 #ifdef __cplusplus
 }

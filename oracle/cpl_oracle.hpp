@@ -78,6 +78,39 @@ CPLO_DIRECTED_BINOP(div_rd, "divsd", mxcsr::rd)
 CPLO_DIRECTED_BINOP(div_ru, "divsd", mxcsr::ru)
 #undef CPLO_DIRECTED_BINOP
 
+// FP32 directed operations: the checkers of the device's FP32 prefilter
+// primitives (single-precision SSE under the same MXCSR modes).
+#define CPLO_DIRECTED_BINOP32(NAME, INSN, MODE)                                \
+  static inline float NAME(float a, float b) {                                \
+    asm("ldmxcsr %[m]\n\t" INSN " %[b], %[a]\n\tldmxcsr %[n]"                 \
+        : [a] "+x"(a)                                                         \
+        : [b] "x"(b), [m] "m"(MODE), [n] "m"(mxcsr::rn));                    \
+    return a;                                                                 \
+  }
+CPLO_DIRECTED_BINOP32(add_rd32, "addss", mxcsr::rd)
+CPLO_DIRECTED_BINOP32(add_ru32, "addss", mxcsr::ru)
+CPLO_DIRECTED_BINOP32(sub_rd32, "subss", mxcsr::rd)
+CPLO_DIRECTED_BINOP32(sub_ru32, "subss", mxcsr::ru)
+CPLO_DIRECTED_BINOP32(mul_rd32, "mulss", mxcsr::rd)
+CPLO_DIRECTED_BINOP32(mul_ru32, "mulss", mxcsr::ru)
+CPLO_DIRECTED_BINOP32(div_rd32, "divss", mxcsr::rd)
+CPLO_DIRECTED_BINOP32(div_ru32, "divss", mxcsr::ru)
+#undef CPLO_DIRECTED_BINOP32
+static inline float f32_rd(double x) {
+  float r;
+  asm("ldmxcsr %[m]\n\tcvtsd2ss %[a], %[r]\n\tldmxcsr %[n]"
+      : [r] "=x"(r)
+      : [a] "x"(x), [m] "m"(mxcsr::rd), [n] "m"(mxcsr::rn));
+  return r;
+}
+static inline float f32_ru(double x) {
+  float r;
+  asm("ldmxcsr %[m]\n\tcvtsd2ss %[a], %[r]\n\tldmxcsr %[n]"
+      : [r] "=x"(r)
+      : [a] "x"(x), [m] "m"(mxcsr::ru), [n] "m"(mxcsr::rn));
+  return r;
+}
+
 static inline double sqrt_rd(double a) {
   CPLO_FLOPS(1);
   double r;

@@ -211,6 +211,16 @@ int cplo_prim(int op, const double* a, const double* b, double* out, size_t n) {
       case 13: r = lower_bound_radius_single_placement(x); break;
       case 14: r = x / y; break;
       case 15: r = std::sqrt(x); break;
+      case 16: r = add_rd32((float)x, (float)y); break;
+      case 17: r = add_ru32((float)x, (float)y); break;
+      case 18: r = sub_rd32((float)x, (float)y); break;
+      case 19: r = sub_ru32((float)x, (float)y); break;
+      case 20: r = mul_rd32((float)x, (float)y); break;
+      case 21: r = mul_ru32((float)x, (float)y); break;
+      case 22: r = div_rd32((float)x, (float)y); break;
+      case 23: r = div_ru32((float)x, (float)y); break;
+      case 27: r = f32_rd(x); break;
+      case 28: r = f32_ru(x); break;
       default: return CPL_ERR_ARG;
     }
     out[i] = r;

@@ -54,7 +54,8 @@ def prim(op, a, b=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     out = np.empty_like(a)
     bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
-    rc = lib().cplo_prim(abi.PRIM_OPS[op], _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size)
+    code = abi.PRIM_OPS[op] if op in abi.PRIM_OPS else abi.PRIM32_OPS[op]
+    rc = lib().cplo_prim(code, _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size)
     assert rc == 0
     return out
 

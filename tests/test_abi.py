@@ -27,12 +27,33 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors against the C compiler's layout of include/cpl.h."""
+    import subprocess
+
     from circlepacking_amd import abi
 
-    assert ctypes.sizeof(abi.Iv) == 16
-    assert ctypes.sizeof(abi.RefineCfg) == 4 * 4 + 8 * 8 + 4 * 5 + 4 * 7
-    # stats: 2 ints, 16 evaluations, 16x8 verdicts, 2x16 digests, 16x6x8 family verdicts,
-    # 16x6 launch seconds, 3 counters, 2 doubles, peak
-    expect = 8 + 8 * (16 + 128 + 32 + 16 * 48) + 8 * 96 + 8 * 3 + 8 * 2 + 8
-    assert ctypes.sizeof(abi.RefineStats) == expect
+    fields = {
+        "cpl_refine_cfg": (abi.RefineCfg, ["grid", "box_lb", "box_ub", "family_mask", "clip", "max_depth",
+                                           "split_mode", "split_axes", "options", "block", "cells_per_thread",
+                                           "prefilter_depths", "reserved"]),
+        "cpl_refine_stats": (abi.RefineStats, [f[0] for f in abi.RefineStats._fields_]),
+        "cpl_tune_point": (abi.TunePoint, ["block", "cells_per_thread", "seconds"]),
+        "cpl_iv": (abi.Iv, ["lb", "ub"]),
+    }
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "cpl.h"', "int main(void) {"]
+    for ctype, (_, names) in fields.items():
+        src.append(f'  printf("{ctype} %zu\\n", sizeof({ctype}));')
+        for n in names:
+            src.append(f'  printf("{ctype}.{n} %zu\\n", offsetof({ctype}, {n}));')
+    src.append("  return 0;\n}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                              check=True).stdout.splitlines())
+    for ctype, (py, names) in fields.items():
+        assert int(got[ctype]) == ctypes.sizeof(py), ctype
+        for n in names:
+            assert int(got[f"{ctype}.{n}"]) == getattr(py, n).offset, (ctype, n)
