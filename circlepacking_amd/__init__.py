@@ -39,6 +39,13 @@ def lib():
         L.cpl_debug_geom.argtypes = [ctypes.c_int, ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int),
                                      ctypes.c_size_t]
         L.cpl_debug_prefilter.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8, _P_U8]
+        _P_F = ctypes.POINTER(ctypes.c_float)
+        L.cpl_mlp_collect.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.c_uint64, ctypes.c_uint32, _P_F, _P_F,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+        L.cpl_mlp_train.argtypes = [_P_F, _P_F, ctypes.c_size_t, _P_F, ctypes.POINTER(abi.MlpTrainCfg), _P_F]
+        L.cpl_mlp_grad.argtypes = [_P_F, _P_F, ctypes.c_size_t, _P_F, _P_F, _P_F, _P_F]
+        L.cpl_mlp_infer.argtypes = [_P_F, ctypes.c_size_t, _P_F, _P_F]
+        L.cpl_mlp_install.argtypes = [ctypes.c_int, _P_F]
         L.cpl_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8]
         L.cpl_autotune.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.c_int, ctypes.POINTER(abi.TunePoint),
                                    ctypes.POINTER(ctypes.c_int)]
@@ -147,3 +154,65 @@ def autotune(cfg, reps=3):
     n = ctypes.c_int()
     _check(lib().cpl_autotune(ctypes.byref(cfg), reps, pts, ctypes.byref(n)))
     return [(p.block, p.cells_per_thread, p.seconds) for p in pts[: n.value]]
+
+
+# ---- learned proposal -------------------------------------------------------
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if cols is not None:
+        a = a.reshape(-1, cols)
+    return a
+
+
+def mlp_collect(cfg, max_records, keep_every=1):
+    """(features (n, 8), targets (n, 8)) float32 records of a plain run of cfg."""
+    feat = np.zeros((int(max_records), 8), dtype=np.float32)
+    targ = np.zeros((int(max_records), 8), dtype=np.float32)
+    n = ctypes.c_uint64()
+    _check(lib().cpl_mlp_collect(ctypes.byref(cfg), int(max_records), int(keep_every), _fptr(feat), _fptr(targ),
+                                 ctypes.byref(n)))
+    return feat[:n.value].copy(), targ[:n.value].copy()
+
+
+def mlp_train(features, targets, params, epochs=5, batch=4096, lr=2e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+              weight_decay=0.0):
+    """Trains on the GPU; returns (trained params, per-epoch mean loss)."""
+    x, t = _f32(features, 8), _f32(targets, 8)
+    w = _f32(params).copy()
+    tc = abi.MlpTrainCfg(epochs, batch, lr, beta1, beta2, eps, weight_decay, 0)
+    losses = np.zeros(epochs, dtype=np.float32)
+    _check(lib().cpl_mlp_train(_fptr(x), _fptr(t), x.shape[0], _fptr(w), ctypes.byref(tc), _fptr(losses)))
+    return w, losses
+
+
+def mlp_grad(features, targets, params, want_outputs=False):
+    """(mean loss, gradient, outputs or None) of the GPU training kernels."""
+    x, t, w = _f32(features, 8), _f32(targets, 8), _f32(params)
+    g = np.zeros(abi.MLP_NPARAM, dtype=np.float32)
+    loss = ctypes.c_float()
+    out = np.zeros((x.shape[0], 8), dtype=np.float32) if want_outputs else None
+    _check(lib().cpl_mlp_grad(_fptr(x), _fptr(t), x.shape[0], _fptr(w), _fptr(g), ctypes.byref(loss),
+                              None if out is None else _fptr(out)))
+    return float(loss.value), g, out
+
+
+def mlp_infer(features, params):
+    """Outputs 0..4 (n, 5) of the in-kernel BF16 MFMA inference."""
+    x, w = _f32(features, 8), _f32(params)
+    out = np.zeros((x.shape[0], 5), dtype=np.float32)
+    _check(lib().cpl_mlp_infer(_fptr(x), x.shape[0], _fptr(w), _fptr(out)))
+    return out
+
+
+def mlp_install(family, params):
+    """Installs (or with None removes) the proposal of one family."""
+    if params is None:
+        _check(lib().cpl_mlp_install(int(family), None))
+    else:
+        w = _f32(params)
+        assert w.size == abi.MLP_NPARAM
+        _check(lib().cpl_mlp_install(int(family), _fptr(w)))

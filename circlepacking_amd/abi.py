@@ -14,7 +14,14 @@ CPL_OK = 0
 ERR_NAMES = {-1: "HIP", -2: "ARG", -3: "TRAP", -4: "NOFIT", -5: "CAPACITY", -6: "NO_DEVICE"}
 MAX_ROUNDS = 16
 
-OPT_GRAPH, OPT_PERSISTENT, OPT_HIER_ROOTS, OPT_FP32, OPT_LEARNED, OPT_NO_DIGEST = 1, 2, 4, 8, 16, 256
+OPT_GRAPH, OPT_PERSISTENT, OPT_HIER_ROOTS, OPT_FP32, OPT_LEARNED, OPT_LEARNED_SPLIT, OPT_NO_DIGEST = 1, 2, 4, 8, 16, 32, 256
+
+MLP_INPUTS, MLP_TARGETS, MLP_NPARAM = 8, 8, 5256
+MLP_H, MLP_OUT = 64, 8
+# parameter offsets: W1[8][64], b1[64], W2[64][64], b2[64], W3[64][8], b3[8]
+MLP_W1, MLP_B1 = 0, 512
+MLP_W2, MLP_B2 = 576, 4672
+MLP_W3, MLP_B3 = 4736, 5248
 
 LAMBDA_MIN = 0.5
 LAMBDA_MAX = 0.9900000000000001
@@ -60,7 +67,8 @@ class RefineCfg(ctypes.Structure):
         ("block", ctypes.c_int32),
         ("cells_per_thread", ctypes.c_int32),
         ("prefilter_depths", ctypes.c_uint32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("learned_threshold", ctypes.c_float),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
     def families(self):
@@ -71,6 +79,12 @@ class RefineCfg(ctypes.Structure):
         for g in self.grid:
             n *= g
         return n
+
+
+class MlpTrainCfg(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("weight_decay", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
 class TunePoint(ctypes.Structure):
@@ -94,10 +108,14 @@ class RefineStats(ctypes.Structure):
         ("eval_seconds", ctypes.c_double),
         ("peak_items", ctypes.c_uint64),
         ("fp32_settled", (ctypes.c_uint64 * 6) * MAX_ROUNDS),
+        ("fp32_tried", (ctypes.c_uint64 * 6) * MAX_ROUNDS),
     ]
 
     def fp32_settled_total(self):
         return sum(self.fp32_settled[d][f] for d in range(MAX_ROUNDS) for f in range(6))
+
+    def fp32_tried_total(self):
+        return sum(self.fp32_tried[d][f] for d in range(MAX_ROUNDS) for f in range(6))
 
     def as_dict(self):
         r = self.rounds
@@ -130,10 +148,12 @@ def make_cfg(grid, box_lb, box_ub, families, clip, max_depth, split_mode, split_
     return c
 
 
-def with_options(cfg, options, block=None, cpt=None, prefilter_depths=None):
+def with_options(cfg, options, block=None, cpt=None, prefilter_depths=None, threshold=None):
     """Copy of cfg with other execution options (and optionally shape)."""
     c = RefineCfg.from_buffer_copy(cfg)
     c.options = int(options)
+    if threshold is not None:
+        c.learned_threshold = float(threshold)
     if prefilter_depths is not None:
         c.prefilter_depths = int(prefilter_depths)
     if block is not None:

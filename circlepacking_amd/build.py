@@ -12,8 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcpl_hip.so")
-SOURCES = ["cpl_hip.hip"]
-HEADERS = ["cpl_round.hpp", "cpl_iv.hpp", "cpl_asin.hpp", "cpl_lane.hpp", "cpl_kernels.hpp"]
+SOURCES = ["cpl_hip.hip", "cpl_mlp.hip"]
+HEADERS = ["cpl_round.hpp", "cpl_iv.hpp", "cpl_asin.hpp", "cpl_lane.hpp", "cpl_kernels.hpp", "cpl_mlp.hpp",
+           "cpl_host.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -38,17 +39,29 @@ def hipcc():
 
 
 def build_hip(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp"] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    """Compiles every source to an object in parallel, then links the library."""
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, "_obj", src.replace(".hip", ".o"))
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + [
+            "-I", os.path.join(ROOT, "include"), "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        objs.append(obj)
     noise = "is not a recognized feature for this target"
-    err = "\n".join(l for l in res.stderr.splitlines() if noise not in l)
+    for src, p in procs:
+        _, stderr = p.communicate()
+        err = "\n".join(l for l in stderr.splitlines() if noise not in l)
+        if p.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src} ({p.returncode}):\n{err}")
+        if err.strip() and verbose:
+            print(err, file=sys.stderr)
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
-    if err.strip() and verbose:
-        print(err, file=sys.stderr)
+        raise RuntimeError(f"link failed ({res.returncode}):\n{res.stderr}")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

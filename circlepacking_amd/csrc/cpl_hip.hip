@@ -15,8 +15,8 @@
 //   k_debug_prim, k_debug_geom, k_debug_prefilter
 //                     primitives, geometry and FP32-prefilter results for the
 //                     parity tests
-// P32 template variants run the FP32 outward-rounded prefilter before the FP64
-// criteria.
+// EXT template variants run the FP32 outward-rounded prefilter before the FP64
+// criteria and the learned proposal (BF16 MFMA) on each wave of cells.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "cpl_host.hpp"
 #include "cpl_kernels.hpp"
 
 using namespace cpl;
@@ -157,17 +158,82 @@ __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, 
   atomicOr(flags + (byte_index >> 2), bits << (8 * (byte_index & 3)));
 }
 
+// Axis of a mode-0 split: the widest eligible axis in root-extent units
+// (smallest grid[a] << splits[a], ties lowest).
+__device__ __forceinline__ int widest_axis(const RefineParams& P, uint32_t meta) {
+  int best = -1;
+  uint64_t bw = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (!(P.split_axes & (1u << a))) continue;
+    uint64_t w = (uint64_t)P.grid[a] << axis_splits(meta, a);
+    if (best < 0 || w < bw) { best = a; bw = w; }
+  }
+  return best;
+}
+
+// Learned proposal of one wave (all 64 lanes call it): outputs 0..4 of each
+// lane's cell.  It runs in a kernel of its own (k_propose) ahead of each round,
+// so that the evaluation kernels keep their register budget.
+__device__ __forceinline__ void propose(const RefineParams& P, bool valid, const double* e, float* y) {
+  float f[8];
+  if (valid) mlp_features(e, P.box_lb, P.box_ub, f);
+  else
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = 0.0f;
+  mlp_infer_wave(P.mlp, f, y);
+}
+
+// Proposal byte of a cell: bit 0 gate (run the FP32 prefilter), bits 1-2 the
+// split axis (the eligible axis of highest predicted score y[1 + a]; the
+// widest axis wins ties and NaN scores), bit 7 set.
+__device__ __forceinline__ uint8_t proposal_byte(const RefineParams& P, const float* y, uint32_t meta) {
+  int axis = widest_axis(P, meta);
+  float ls = y[1 + axis];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    if ((P.split_axes & (1u << a)) && y[1 + a] > ls) { axis = a; ls = y[1 + a]; }
+  return (uint8_t)(0x80u | (uint32_t)(axis << 1) | (y[0] > P.mlp_threshold ? 1u : 0u));
+}
+// Without a proposal: gate open, widest axis.
+constexpr uint8_t kNoProposal = 0x01;
+
+__device__ __forceinline__ int split_axis(const RefineParams& P, uint32_t meta, uint8_t pb) {
+  return (P.learned_split && (pb & 0x80)) ? (int)((pb >> 1) & 3) : widest_axis(P, meta);
+}
+
+// Proposal bytes of a round's input items (before the round runs on them).
+__global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, uint64_t n, uint8_t* out) {
+  if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    double e[8];
+    uint32_t meta = 0;
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
+      meta = in.meta[i];
+    }
+    float y[5];
+    propose(P, valid, e, y);
+    if (valid) out[i] = proposal_byte(P, y, meta);
+  }
+}
+
 // One round of family F.  ROOT: items are the implicit root grid (n = n_roots),
 // else items come from `in` (n items).
 // LB: launch bound (maximum threads per block) the register allocation is made for.
-// P32: FP32 prefilter kernels (enabled per depth by P.prefilter_depths).
-template <int F, bool ROOT, int LB = 256, bool P32 = false>
+// EXT: extended kernels with the FP32 prefilter (enabled per depth by
+// P.prefilter_depths) and the learned proposal (P.mlp).
+template <int F, bool ROOT, int LB = 256, bool EXT = false>
 __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint64_t n) {
   if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
-  unsigned long long c_f32 = 0;
+  unsigned long long c_f32 = 0, c_f32t = 0;
 
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -200,14 +266,19 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
         la_idx = in.la_idx[i];
         meta = in.meta[i];
       }
+    }
+    uint8_t pb = kNoProposal;
+    if (EXT && !ROOT && P.prop && valid) pb = P.prop[i];
+    if (valid) {
       int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      bool hit = false;
-      const bool pre = P32 && ((P.prefilter_depths >> depth) & 1u);
-      vb = eval_synthetic<F, P32>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp,
-                                  pre, &hit);
+      bool hit = false, tried = false;
+      const bool pre = EXT && ((P.prefilter_depths >> depth) & 1u) && (pb & 1);
+      vb = eval_synthetic<F, EXT>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp,
+                                  pre, &hit, &tried);
       c_f32 += hit ? 1 : 0;
+      c_f32t += tried ? 1 : 0;
       int v = vb & 7;
       c_eval += 1;
       c_v[v] += 1;
@@ -249,15 +320,7 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
           int depth = (int)(meta >> 16);
           uint32_t splits = meta & 0xFFFFu;
           if (P.split_mode == 0) {
-            // widest eligible axis in root-extent units: smallest grid[a] << splits[a], ties lowest
-            int best = -1;
-            uint64_t bw = 0;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              if (!(P.split_axes & (1u << a))) continue;
-              uint64_t w = (uint64_t)P.grid[a] << axis_splits(meta, a);
-              if (best < 0 || w < bw) { best = a; bw = w; }
-            }
+            const int best = split_axis(P, meta, pb);
             IV s0 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 0, 2);
             IV s1 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 1, 2);
             uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | (splits + (1u << (4 * best)));
@@ -313,7 +376,10 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
   wave_add(&S->vdigest, c_vdig);
   wave_add(&S->certified, c_cert);
   wave_add(&S->unresolved, c_unres);
-  if (P32) wave_add(&S->fp32_settled, c_f32);
+  if (EXT) {
+    wave_add(&S->fp32_settled, c_f32);
+    wave_add(&S->fp32_tried, c_f32t);
+  }
 }
 
 // Root round with admissibility decided once per (la, r1, r2) triple: an
@@ -442,24 +508,20 @@ struct PersistParams {
 };
 
 enum : uint32_t { Q_ROOT = 0, Q_ERR = 4 };
-enum : int { ST_EVAL = 0, ST_V = 1, ST_DIG = 9, ST_VDIG = 10, ST_CERT = 11, ST_UNRES = 12, ST_F32 = 13, ST_N = 14 };
+enum : int {
+  ST_EVAL = 0, ST_V = 1, ST_DIG = 9, ST_VDIG = 10, ST_CERT = 11, ST_UNRES = 12, ST_F32 = 13, ST_F32T = 14, ST_N = 15
+};
 
 // Child c of a cell under the refinement rule (same rule as k_round).
 __device__ __forceinline__ void make_child(const RefineParams& P, const double e[8], uint32_t meta, uint32_t la_idx,
-                                           int c, double ce[8], uint32_t& cmeta, uint32_t& cla) {
+                                           int axis, int c, double ce[8], uint32_t& cmeta, uint32_t& cla) {
   const int depth = (int)(meta >> 16);
   const uint32_t splits = meta & 0xFFFFu;
 #pragma unroll
   for (int k = 0; k < 8; ++k) ce[k] = e[k];
   cla = la_idx;
   if (P.split_mode == 0) {
-    int best = -1;
-    uint64_t bw = 0;
-    for (int a = 0; a < 4; ++a) {
-      if (!(P.split_axes & (1u << a))) continue;
-      uint64_t w = (uint64_t)P.grid[a] << axis_splits(meta, a);
-      if (best < 0 || w < bw) { best = a; bw = w; }
-    }
+    const int best = axis;
     IV sub = get_subinterval(IV(e[2 * best], e[2 * best + 1]), c, 2);
     ce[2 * best] = sub.lb;
     ce[2 * best + 1] = sub.ub;
@@ -483,7 +545,96 @@ __device__ __forceinline__ void make_child(const RefineParams& P, const double e
   }
 }
 
-template <int F, bool P32 = false>
+// ---- training records of the learned proposal ----
+struct RecordSink {
+  float* feat;       // [cap][8]
+  float* targ;       // [cap][8]
+  uint32_t* count;
+  uint64_t cap;
+  uint32_t keep;     // keep one item in `keep` (by item hash)
+};
+
+template <bool ROOT>
+__device__ __forceinline__ void load_item(const RefineParams& P, const Worklist& in, uint64_t i, double* e,
+                                          uint32_t& root, uint32_t& la_idx, uint32_t& meta) {
+  if (ROOT) {
+    uint64_t r = i;
+    int idx[4];
+#pragma unroll
+    for (int a = 3; a >= 0; --a) { idx[a] = (int)(r % (uint64_t)P.grid[a]); r /= (uint64_t)P.grid[a]; }
+    e[0] = P.la_table[3 * idx[0] + 0];
+    e[1] = P.la_table[3 * idx[0] + 1];
+#pragma unroll
+    for (int a = 1; a < 4; ++a) {
+      IV s = get_subinterval(IV(P.box_lb[a], P.box_ub[a]), idx[a], P.grid[a]);
+      e[2 * a] = s.lb;
+      e[2 * a + 1] = s.ub;
+    }
+    root = (uint32_t)i;
+    la_idx = (uint32_t)idx[0];
+    meta = 0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
+    root = in.root[i];
+    la_idx = in.la_idx[i];
+    meta = in.meta[i];
+  }
+}
+
+__device__ __forceinline__ double item_lbrsp(const RefineParams& P, uint32_t meta, uint32_t la_idx) {
+  return P.clip ? P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2] : 0.0;
+}
+
+// One record per kept item that reaches the criteria: features, whether the
+// FP32 prefilter proves it, and (CRITICAL items that will be split) the
+// settled fraction of the children of each eligible axis.
+template <int F, bool ROOT>
+__global__ __launch_bounds__(256) void k_record(RefineParams P, Worklist in, uint64_t n, RecordSink S) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double e[8];
+    uint32_t root, la_idx, meta;
+    load_item<ROOT>(P, in, i, e, root, la_idx, meta);
+    if (S.keep > 1 && mix64(item_hash(e, root, F, meta)) % S.keep != 0) continue;
+    const int depth = (int)(meta >> 16);
+    const double lbrsp = item_lbrsp(P, meta, la_idx);
+    const IV la(e[0], e[1]), r1(e[2], e[3]), r2(e[4], e[5]), p3(e[6], e[7]);
+    const int v = eval_synthetic<F>(la, r1, r2, p3, P.clip != 0, lbrsp) & 7;
+    if (v != V_PROVEN && v != V_CRITICAL && v != V_TRAP) continue;
+    bool hit = false;
+    (void)eval_synthetic<F, true>(la, r1, r2, p3, P.clip != 0, lbrsp, true, &hit);
+    float t[8] = {hit ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, (float)v, (float)depth};
+    if (v == V_CRITICAL && depth < P.max_depth && P.split_mode == 0) {
+      t[5] = 1.0f;
+      for (int a = 0; a < 4; ++a) {
+        if (!(P.split_axes & (1u << a))) continue;
+        int settled = 0;
+        for (int c = 0; c < 2; ++c) {
+          double ce[8];
+          uint32_t cmeta, cla;
+          make_child(P, e, meta, la_idx, a, c, ce, cmeta, cla);
+          const int cv = eval_synthetic<F>(IV(ce[0], ce[1]), IV(ce[2], ce[3]), IV(ce[4], ce[5]), IV(ce[6], ce[7]),
+                                           P.clip != 0, item_lbrsp(P, cmeta, cla)) & 7;
+          settled += (cv == V_PROVEN || cv == V_EMPTY || cv == V_OOD) ? 1 : 0;
+        }
+        t[1 + a] = 0.5f * (float)settled;
+      }
+    }
+    float f[8];
+    mlp_features(e, P.box_lb, P.box_ub, f);
+    const uint32_t pos = atomicAdd(S.count, 1u);
+    if (pos < S.cap) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        S.feat[(size_t)pos * 8 + k] = f[k];
+        S.targ[(size_t)pos * 8 + k] = t[k];
+      }
+    }
+  }
+}
+
+template <int F, bool EXT = false>
 __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
   extern __shared__ __align__(16) char lds[];
   const RefineParams& P = Q.R;
@@ -543,18 +694,20 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
     }
     count = base;
     bool split = false;
+    const uint8_t pb = kNoProposal;  // the learned proposal runs with the round-based forms only
     if (valid) {
       const int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      bool hit = false;
-      const bool pre = P32 && ((P.prefilter_depths >> depth) & 1u);
-      uint8_t vb = eval_synthetic<F, P32>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0,
-                                          lbrsp, pre, &hit);
+      bool hit = false, tried = false;
+      const bool pre = EXT && ((P.prefilter_depths >> depth) & 1u) && (pb & 1);
+      uint8_t vb = eval_synthetic<F, EXT>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0,
+                                          lbrsp, pre, &hit, &tried);
       const int v = vb & 7;
       unsigned long long* st = sst + depth * ST_N;
       atomicAdd(st + ST_EVAL, 1ull);
       if (hit) atomicAdd(st + ST_F32, 1ull);
+      if (tried) atomicAdd(st + ST_F32T, 1ull);
       atomicAdd(st + ST_V + v, 1ull);
       if (P.digest) {
         uint64_t h = item_hash(e, root, F, meta);
@@ -583,10 +736,11 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
     if (mask) {
       const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
       if (split) {
+        const int axis = P.split_mode == 0 ? split_axis(P, meta, pb) : 0;
         for (int c = 0; c < NCH; ++c) {
           double ce[8];
           uint32_t cmeta, cla;
-          make_child(P, e, meta, la_idx, c, ce, cmeta, cla);
+          make_child(P, e, meta, la_idx, axis, c, ce, cmeta, cla);
           const int o = count + rank * NCH + c;
           for (int k = 0; k < 8; ++k) se[k * CAP + o] = ce[k];
           su[0 * CAP + o] = root;
@@ -623,6 +777,7 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
                             : c == ST_VDIG ? &S->vdigest
                             : c == ST_CERT ? &S->certified
                             : c == ST_F32  ? &S->fp32_settled
+                            : c == ST_F32T ? &S->fp32_tried
                                            : &S->unresolved;
     atomicAdd(dst, v);
   }
@@ -746,40 +901,9 @@ __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
 // ============================================================================
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t _e = (expr);                                                                    \
-    if (_e != hipSuccess)                                                                      \
-      return fail(CPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));              \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  int ensure(size_t need) {
-    if (need <= bytes) return CPL_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    hipError_t e = hipMalloc(&p, need);
-    if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    bytes = need;
-    return CPL_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
+using cpl_host::DevBuf;
+using cpl_host::fail;
+using cpl_host::g_err;
 
 struct Context {
   bool ready = false;
@@ -793,6 +917,12 @@ struct Context {
   DevBuf la_table, root_flags, stats, counters;
   DevBuf mr_ranges, mr_misc, mr_crit;
   DevBuf persist_q;
+  DevBuf mlp;                 // MlpPacked per family
+  DevBuf prop;                // proposal bytes of a round's input, one region per family
+  bool mlp_ready[CPL_NUM_FAMILIES] = {};
+  DevBuf records;             // training records being collected
+  bool recording = false;
+  RecordSink sink{};
   uint64_t wl_cap_items = 0;  // per family region
 };
 Context g;
@@ -815,6 +945,13 @@ int ensure_init() {
   g.ready = true;
   return CPL_OK;
 }
+
+}  // namespace
+
+int cpl_host::ensure_init() { return ::ensure_init(); }
+hipStream_t cpl_host::stream() { return g.stream; }
+
+namespace {
 
 // Stream the refinement launch helpers enqueue on (the library stream, or a
 // family branch while forking).
@@ -852,9 +989,9 @@ template <int F> struct LaunchBody {
   }
 };
 template <int F> struct LaunchRoundB {
-  template <bool R, int LB, bool P32>
+  template <bool R, int LB, bool EXT>
   static void go(unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
-    hipLaunchKernelGGL((k_round<F, R, LB, P32>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
+    hipLaunchKernelGGL((k_round<F, R, LB, EXT>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
   }
   static void run(bool root, bool p32, unsigned grid, int block, const RefineParams& P, const Worklist& in,
                   uint64_t n) {
@@ -879,6 +1016,12 @@ template <int F> struct LaunchRound {
 template <int F> struct LaunchPrefilterDebug {
   static void run(unsigned grid, const double* cells, size_t n, uint8_t* o32, uint8_t* o64) {
     hipLaunchKernelGGL(k_debug_prefilter<F>, dim3(grid), dim3(256), 0, g.stream, cells, n, o32, o64);
+  }
+};
+template <int F> struct LaunchRecord {
+  static void run(bool root, unsigned grid, const RefineParams& P, const Worklist& in, uint64_t n, const RecordSink& S) {
+    if (root) hipLaunchKernelGGL((k_record<F, true>), dim3(grid), dim3(256), 0, g.stream, P, in, n, S);
+    else hipLaunchKernelGGL((k_record<F, false>), dim3(grid), dim3(256), 0, g.stream, P, in, n, S);
   }
 };
 template <int F> struct LaunchModeR {
@@ -913,7 +1056,7 @@ int cpl_shutdown(void) {
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf* b : {&g.scratch_in, &g.scratch_in2, &g.scratch_out, &g.cells, &g.verdicts, &g.wl[0], &g.wl[1],
                     &g.la_table, &g.root_flags, &g.stats, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit,
-                    &g.persist_q})
+                    &g.persist_q, &g.mlp, &g.records, &g.prop})
     b->release();
   (void)hipStreamDestroy(g.stream);
   g.stream = nullptr;
@@ -1034,7 +1177,8 @@ struct Plan {
   uint64_t cap = 0;
   bool digest = true;
   bool flags = false;
-  bool p32 = false;
+  bool ext = false;       // extended kernels (prefilter and / or learned proposal)
+  bool learned = false;
   uint32_t prefilter_depths = 0;
   int block = 256;
   int cpt = 1;
@@ -1063,8 +1207,12 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
-  pl.p32 = (cfg->options & CPL_OPT_FP32) != 0;
-  pl.prefilter_depths = cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
+  pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT)) != 0;
+  pl.ext = (cfg->options & CPL_OPT_FP32) != 0 || pl.learned;
+  pl.prefilter_depths = !(cfg->options & CPL_OPT_FP32) ? 0u : cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
+  if (pl.learned)
+    for (int f : pl.fams)
+      if (!g.mlp_ready[f]) return fail(CPL_ERR_ARG, "learned options need a proposal installed for every family (cpl_mlp_install)");
   pl.block = cfg->block > 0 ? cfg->block : 256;
   if (pl.block % 64 || pl.block > 1024) return fail(CPL_ERR_ARG, "block must be a multiple of 64 up to 1024");
   pl.cpt = cfg->cells_per_thread > 0 ? cfg->cells_per_thread : 1;
@@ -1084,6 +1232,7 @@ int ensure_buffers(const Plan& pl) {
   if ((rc = g.counters.ensure(4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
   for (int b = 0; b < 2; ++b)
     if ((rc = g.wl[b].ensure((size_t)pl.cap * per_item * pl.nf))) return rc;
+  if (pl.learned && (rc = g.prop.ensure((size_t)pl.cap * pl.nf))) return rc;
   g.wl_cap_items = pl.cap;
   return CPL_OK;
 }
@@ -1113,7 +1262,25 @@ RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   P.out_cap = pl.cap;
   P.digest = pl.digest ? 1 : 0;
   P.prefilter_depths = pl.prefilter_depths;
+  P.mlp_threshold = cfg->learned_threshold;
+  P.learned_split = (cfg->options & CPL_OPT_LEARNED_SPLIT) ? 1 : 0;
   return P;
+}
+
+// Per-launch family fields: slot and learned proposal.
+void set_family(RefineParams& P, const Plan& pl, int s) {
+  P.fam_slot = s;
+  P.mlp = pl.learned ? g.mlp.as<MlpPacked>() + pl.fams[s] : nullptr;
+}
+
+// Proposal bytes of the items of `in` (n, or *P.in_count when set) into the
+// family's proposal region; the round launched next reads them through P.prop.
+void launch_propose(const Plan& pl, int s, RefineParams& P, const Worklist& in, uint64_t n, unsigned grid) {
+  P.prop = nullptr;
+  if (!pl.learned) return;
+  uint8_t* out = g.prop.as<uint8_t>() + (size_t)s * pl.cap;
+  hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, launch_stream(), P, in, n, out);
+  P.prop = out;
 }
 
 void fill(void* p, uint64_t words, uint32_t v) {
@@ -1150,6 +1317,7 @@ int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
       stats->certified += r.certified;
       stats->unresolved += r.unresolved;
       stats->fp32_settled[d][f] = r.fp32_settled;
+      stats->fp32_tried[d][f] = r.fp32_tried;
       stats->total_evaluations += r.evaluations;
     }
     if (stats->evaluations[d]) stats->rounds = d + 1;
@@ -1188,15 +1356,21 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
       uint64_t n = d == 0 ? pl.n_roots : cur_n[s];
       if (n == 0) continue;
       any = true;
-      P.fam_slot = s;
+      set_family(P, pl, s);
       P.out = region(g.wl[nxt], pl.cap, s);
       P.out_count = counter(nxt, s);
       Worklist in = region(g.wl[cur], pl.cap, s);
       hipEvent_t a, b;
       HIP_TRY(hipEventCreate(&a));
       HIP_TRY(hipEventCreate(&b));
+      if (g.recording) {
+        dispatch_family<LaunchRecord>(pl.fams[s], d == 0, grid_for(n, 256), P, in, n, g.sink);
+        HIP_TRY(hipGetLastError());
+      }
+      P.prop = nullptr;
+      if (d > 0) launch_propose(pl, s, P, in, n, grid_for(n, 256));
       HIP_TRY(hipEventRecord(a, g.stream));
-      dispatch_family<LaunchRound>(pl.fams[s], d == 0, pl.p32, grid_for(n, 256), P, in, n);
+      dispatch_family<LaunchRound>(pl.fams[s], d == 0, pl.ext, grid_for(n, 256), P, in, n);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipEventRecord(b, g.stream));
       kev.push_back(a);
@@ -1297,7 +1471,7 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
   fork_families(pl.nf);
   for (int s = 0; s < pl.nf; ++s) {
     g_launch_stream = g.fam_stream[s];
-    P.fam_slot = s;
+    set_family(P, pl, s);
     P.depth = 0;
     P.in_count = nullptr;
     Worklist none{};
@@ -1311,12 +1485,14 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
       P.in_count = counter(2, s);
       P.out = region(g.wl[1], pl.cap, s);
       P.out_count = counter(1, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.p32, grid_tuned(exp_of(0, s), pl), pl.block, P,
+      P.prop = nullptr;  // the root round runs without proposals in every form
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(0, s), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0);
     } else {
       P.out = region(g.wl[1], pl.cap, s);
       P.out_count = counter(1, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.p32, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
+      P.prop = nullptr;
+      dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.ext, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
     }
     int cur = 1;
     for (int d = 1; d <= cfg->max_depth; ++d) {
@@ -1326,7 +1502,8 @@ void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<ui
       P.in_count = counter(cur, s);
       P.out = region(g.wl[nxt], pl.cap, s);
       P.out_count = counter(nxt, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.p32, grid_tuned(exp_of(d, s), pl), pl.block, P,
+      launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, grid_tuned(exp_of(d, s), pl));
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d, s), pl), pl.block, P,
                                     region(g.wl[cur], pl.cap, s), (uint64_t)0);
       cur = nxt;
     }
@@ -1413,12 +1590,12 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
   for (int f : pl.fams) {
     int gsz = 0;
     switch (f) {
-      case 0: gsz = LaunchPersist<0>::grid_for_lds(lds, pl.p32); break;
-      case 1: gsz = LaunchPersist<1>::grid_for_lds(lds, pl.p32); break;
-      case 2: gsz = LaunchPersist<2>::grid_for_lds(lds, pl.p32); break;
-      case 3: gsz = LaunchPersist<3>::grid_for_lds(lds, pl.p32); break;
-      case 4: gsz = LaunchPersist<4>::grid_for_lds(lds, pl.p32); break;
-      default: gsz = LaunchPersist<5>::grid_for_lds(lds, pl.p32); break;
+      case 0: gsz = LaunchPersist<0>::grid_for_lds(lds, pl.ext); break;
+      case 1: gsz = LaunchPersist<1>::grid_for_lds(lds, pl.ext); break;
+      case 2: gsz = LaunchPersist<2>::grid_for_lds(lds, pl.ext); break;
+      case 3: gsz = LaunchPersist<3>::grid_for_lds(lds, pl.ext); break;
+      case 4: gsz = LaunchPersist<4>::grid_for_lds(lds, pl.ext); break;
+      default: gsz = LaunchPersist<5>::grid_for_lds(lds, pl.ext); break;
     }
     grid_max = std::max(grid_max, gsz);
   }
@@ -1444,7 +1621,7 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
   fork_families(pl.nf);
   for (int s = 0; s < pl.nf; ++s) {
     g_launch_stream = g.fam_stream[s];
-    P.fam_slot = s;
+    set_family(P, pl, s);
     RefineParams T = P;
     T.out = region(g.wl[0], pl.cap, s);
     T.out_count = counter(2, s);
@@ -1462,12 +1639,12 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
     Q.cap = cap_lds;
     Q.claim = (uint32_t)std::min(64, 8 * pl.cpt);
     switch (pl.fams[s]) {
-      case 0: LaunchPersist<0>::run(LaunchPersist<0>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
-      case 1: LaunchPersist<1>::run(LaunchPersist<1>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
-      case 2: LaunchPersist<2>::run(LaunchPersist<2>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
-      case 3: LaunchPersist<3>::run(LaunchPersist<3>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
-      case 4: LaunchPersist<4>::run(LaunchPersist<4>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
-      default: LaunchPersist<5>::run(LaunchPersist<5>::grid_for_lds(lds, pl.p32), lds, pl.p32, Q); break;
+      case 0: LaunchPersist<0>::run(LaunchPersist<0>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
+      case 1: LaunchPersist<1>::run(LaunchPersist<1>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
+      case 2: LaunchPersist<2>::run(LaunchPersist<2>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
+      case 3: LaunchPersist<3>::run(LaunchPersist<3>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
+      case 4: LaunchPersist<4>::run(LaunchPersist<4>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
+      default: LaunchPersist<5>::run(LaunchPersist<5>::grid_for_lds(lds, pl.ext), lds, pl.ext, Q); break;
     }
     HIP_TRY(hipGetLastError());
   }
@@ -1502,12 +1679,64 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   Plan pl;
   if ((rc = make_plan(cfg, root_flags != nullptr, pl))) return rc;
   if ((rc = ensure_buffers(pl))) return rc;
-  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_FP32 | CPL_OPT_NO_DIGEST;
+  const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_FP32 | CPL_OPT_LEARNED |
+                         CPL_OPT_LEARNED_SPLIT | CPL_OPT_NO_DIGEST;
   if (cfg->options & ~known) return fail(CPL_ERR_ARG, "unsupported option bits");
+  if ((cfg->options & CPL_OPT_PERSISTENT) && pl.learned)
+    return fail(CPL_ERR_ARG, "the learned proposal runs with the round-based forms (plain or CPL_OPT_GRAPH)");
   if (cfg->options & CPL_OPT_PERSISTENT) return refine_persistent(cfg, pl, stats, root_flags);
   if (cfg->options & CPL_OPT_GRAPH) return refine_graph(cfg, pl, stats, root_flags);
   if (cfg->options & CPL_OPT_HIER_ROOTS) return fail(CPL_ERR_ARG, "CPL_OPT_HIER_ROOTS needs CPL_OPT_GRAPH");
   return refine_plain(cfg, pl, stats, root_flags);
+}
+
+int cpl_mlp_install(int family, const float* params) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (family < 0 || family >= CPL_NUM_FAMILIES) return fail(CPL_ERR_ARG, "cpl_mlp_install: bad family");
+  if (!params) {
+    g.mlp_ready[family] = false;
+    return CPL_OK;
+  }
+  if ((rc = g.mlp.ensure(CPL_NUM_FAMILIES * sizeof(MlpPacked)))) return rc;
+  std::vector<char> host(sizeof(MlpPacked));
+  cpl_mlp_pack(params, host.data());
+  HIP_TRY(hipMemcpyAsync(g.mlp.as<MlpPacked>() + family, host.data(), host.size(), hipMemcpyHostToDevice, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  g.mlp_ready[family] = true;
+  return CPL_OK;
+}
+
+int cpl_mlp_collect(const cpl_refine_cfg* cfg, uint64_t max_records, uint32_t keep_every, float* features,
+                    float* targets, uint64_t* n_records) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!cfg || !features || !targets || !n_records || max_records == 0 || max_records > (1ull << 28))
+    return fail(CPL_ERR_ARG, "cpl_mlp_collect: bad argument");
+  cpl_refine_cfg c = *cfg;
+  c.options = CPL_OPT_NO_DIGEST;
+  Plan pl;
+  if ((rc = make_plan(&c, false, pl))) return rc;
+  if ((rc = ensure_buffers(pl))) return rc;
+  if ((rc = g.records.ensure(max_records * 16 * sizeof(float) + 256))) return rc;
+  g.sink.feat = g.records.as<float>();
+  g.sink.targ = g.records.as<float>() + max_records * 8;
+  g.sink.count = reinterpret_cast<uint32_t*>(g.records.as<float>() + max_records * 16);
+  g.sink.cap = max_records;
+  g.sink.keep = keep_every > 0 ? keep_every : 1;
+  HIP_TRY(hipMemsetAsync(g.sink.count, 0, sizeof(uint32_t), g.stream));
+  g.recording = true;
+  cpl_refine_stats st;
+  rc = refine_plain(&c, pl, &st, nullptr);
+  g.recording = false;
+  if (rc) return rc;
+  uint32_t cnt = 0;
+  HIP_TRY(hipMemcpy(&cnt, g.sink.count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  const uint64_t n = std::min<uint64_t>(cnt, max_records);
+  HIP_TRY(hipMemcpy(features, g.sink.feat, n * 8 * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(targets, g.sink.targ, n * 8 * sizeof(float), hipMemcpyDeviceToHost));
+  *n_records = n;
+  return CPL_OK;
 }
 
 int cpl_autotune(cpl_refine_cfg* cfg, int reps, cpl_tune_point* points, int* n_points) {

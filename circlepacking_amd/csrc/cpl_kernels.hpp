@@ -2,6 +2,7 @@
 #pragma once
 
 #include "cpl_lane.hpp"
+#include "cpl_mlp.hpp"
 #include "../../include/cpl.h"
 
 namespace cpl {
@@ -27,7 +28,8 @@ struct RoundStats {
   unsigned long long certified;
   unsigned long long unresolved;
   unsigned long long fp32_settled;  // items the FP32 prefilter proved
-  unsigned long long pad[2];
+  unsigned long long fp32_tried;    // items the FP32 prefilter ran on
+  unsigned long long pad;
 };
 
 struct RefineParams {
@@ -51,7 +53,11 @@ struct RefineParams {
   uint32_t* overflow;
   const uint32_t* in_count;  // device-side item count of the input (graph mode), or null
   int32_t digest;            // 1: accumulate the per-item digests
-  uint32_t prefilter_depths; // FP32 prefilter kernels: bit d enables it at depth d
+  uint32_t prefilter_depths; // extended kernels: bit d enables the FP32 prefilter at depth d
+  const MlpPacked* mlp;      // extended kernels: learned proposal of this family, or null
+  float mlp_threshold;       // gate: the prefilter runs when output 0 exceeds this logit
+  int32_t learned_split;     // 1: mode-0 splits take the proposal's best-scored eligible axis
+  const uint8_t* prop;       // extended round kernels: proposal byte per input item (k_propose), or null
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {

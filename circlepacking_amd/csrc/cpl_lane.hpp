@@ -499,10 +499,10 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
 // PREFILTER (and `pre` at run time): the criteria are first evaluated on the
 // FP32 outward enclosure of the tightened cell; a cell FP32 proves is PROVEN,
 // every other cell is decided by the FP64 evaluation.  `*fp32_hit` is set when
-// FP32 settled the cell.
+// FP32 settled the cell, `*fp32_tried` when the FP32 criteria ran.
 template <int F, bool PREFILTER = false>
 __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3, bool pre = true,
-                                                    bool* fp32_hit = nullptr) {
+                                                    bool* fp32_hit = nullptr, bool* fp32_tried = nullptr) {
   IV r2 = r2_;
   if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
   IV r1 = r1_;
@@ -517,6 +517,7 @@ __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3
   }
   if constexpr (PREFILTER) {
     if (pre) {
+      if (fp32_tried) *fp32_tried = true;
       int c32 = criteria<F, float>(to_f32(la), to_f32(r1), to_f32(r2), to_f32(r3));
       if (c32 == 0) {
         if (fp32_hit) *fp32_hit = true;
@@ -542,7 +543,8 @@ __device__ __forceinline__ IV r3_from_position(IV range, IV p) {
 // Synthetic cell (la, r1, r2, p3) of family F; lbrsp = lower_bound_radius_single_placement(la.lb).
 template <int F, bool PREFILTER = false>
 __device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp,
-                                                  bool pre = true, bool* fp32_hit = nullptr) {
+                                                  bool pre = true, bool* fp32_hit = nullptr,
+                                                  bool* fp32_tried = nullptr) {
   IV r1c = r1, r2c = r2;
   if (clip) {
     r1c = r1.intersect(compute_r1_range(la, lbrsp));
@@ -556,7 +558,7 @@ __device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bo
     if (range.empty()) return vbyte(V_OOD, 0, 0);
     r3 = r3_from_position(range, p3);
   }
-  return eval_kernel_body<F, PREFILTER>(la, r1c, r2c, r3, pre, fp32_hit);
+  return eval_kernel_body<F, PREFILTER>(la, r1c, r2c, r3, pre, fp32_hit, fp32_tried);
 }
 
 }  // namespace cpl

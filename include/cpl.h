@@ -97,18 +97,24 @@ typedef struct {
   int32_t block;              /* threads per block of the evaluation kernels (0: 256) */
   int32_t cells_per_thread;   /* items a thread takes per scheduling step (0: 1) */
   uint32_t prefilter_depths;  /* with CPL_OPT_FP32: bit d enables the prefilter at depth d (0: every depth) */
-  int32_t reserved[3];
+  float learned_threshold;    /* with CPL_OPT_LEARNED: the prefilter runs on cells whose gate logit exceeds this */
+  int32_t reserved[2];
 } cpl_refine_cfg;
 
-/* Execution options of cpl_refine_run.  Every option changes scheduling or
- * precision of intermediate filtering only: verdicts, counts and root flags are
- * identical to the plain path. */
+/* Execution options of cpl_refine_run.  Every option except
+ * CPL_OPT_LEARNED_SPLIT changes scheduling or the precision of intermediate
+ * filtering only: verdicts, counts and root flags are identical to the plain
+ * path.  CPL_OPT_LEARNED_SPLIT changes which cells are evaluated (another
+ * bisection order); each verdict is still the interval evaluation's. */
 enum {
   CPL_OPT_GRAPH = 1,       /* all rounds enqueued with device-side sizes, captured once as a hipGraph, replayed */
   CPL_OPT_PERSISTENT = 2,  /* one persistent kernel drains a mixed-depth queue (LDS-staged batches) */
   CPL_OPT_HIER_ROOTS = 4,  /* decide admissibility once per (la, r1, r2) root triple */
   CPL_OPT_FP32 = 8,        /* FP32 outward-rounded prefilter; FP64 certifies what it leaves undecided */
-  CPL_OPT_LEARNED = 16,    /* MLP proposal orders criteria / split axes (the interval check decides) */
+  CPL_OPT_LEARNED = 16,    /* MLP proposal decides per cell whether the FP32 screen runs before the FP64 criteria */
+  CPL_OPT_LEARNED_SPLIT = 32, /* mode-0 splits take the MLP's best-scored axis; changes the leaf set, not soundness */
+  /* The learned options apply from depth 1 on (the root round runs without a
+   * proposal) and run with the plain and CPL_OPT_GRAPH forms. */
   CPL_OPT_NO_DIGEST = 256  /* skip the per-item digests (timing runs) */
 };
 
@@ -139,6 +145,7 @@ typedef struct {
   double eval_seconds;                             /* device time of the criteria kernels only */
   uint64_t peak_items;                             /* largest worklist held on the device */
   uint64_t fp32_settled[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES]; /* items the FP32 prefilter proved (CPL_OPT_FP32) */
+  uint64_t fp32_tried[CPL_MAX_ROUNDS][CPL_NUM_FAMILIES];   /* items the FP32 prefilter ran on */
 } cpl_refine_stats;
 
 /* ---- lifecycle --------------------------------------------------------- */
@@ -215,6 +222,51 @@ int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, s
  * first, 16 + trap code; 255 when the cell is decided before the criteria)
  * and the FP64 verdict byte. */
 int cpl_debug_prefilter(int family, const double* cells, size_t n, uint8_t* fp32_result, uint8_t* verdicts);
+
+/* ---- learned proposal (no reference counterpart) ------------------------- */
+/* An 8 -> 64 -> 64 -> 8 ReLU network per family.  Parameters, FP32, in this
+ * order: W1[8][64], b1[64], W2[64][64], b2[64], W3[64][8], b3[8] (W[i][j]
+ * connects input i to unit j).  Output 0: logit of "the FP32 prefilter proves
+ * the cell"; outputs 1..4: predicted one-step settle score of splitting axis
+ * la, r1, r2, p3. */
+#define CPL_MLP_INPUTS 8
+#define CPL_MLP_TARGETS 8
+#define CPL_MLP_NPARAM 5256
+
+/* Training records of a refinement run: every evaluated (cell, family) item
+ * that reaches the criteria (PROVEN, CRITICAL or TRAP), one in keep_every by a
+ * hash of the item.  features: n x 8 floats (per axis: midpoint mapped to
+ * [-1, 1] over the box, log2(width / box extent) / 16 + 1 clamped to [-1, 1]);
+ * targets:
+ * n x 8 floats (0: FP32 prefilter proves the cell, 1..4: fraction of the two
+ * children settled when splitting axis a (mode-0 runs, CRITICAL cells above
+ * max depth), 5: 1 where 1..4 are defined, 6: FP64 verdict, 7: depth).
+ * Runs the plain path; family_mask selects the families (records of one
+ * family when one bit is set). */
+int cpl_mlp_collect(const cpl_refine_cfg* cfg, uint64_t max_records, uint32_t keep_every, float* features,
+                    float* targets, uint64_t* n_records);
+
+typedef struct {
+  int32_t epochs, batch;
+  float lr, beta1, beta2, eps, weight_decay;
+  int32_t reserved;
+} cpl_mlp_train_cfg;
+
+/* Trains params (in: initial, out: trained) with Adam on the GPU over n records
+ * in the given order, minibatches of cfg->batch; epoch_loss (may be NULL):
+ * cfg->epochs mean losses.  Loss: BCE on output 0 plus targets[5] * mean
+ * squared error of outputs 1..4. */
+int cpl_mlp_train(const float* features, const float* targets, size_t n, float* params,
+                  const cpl_mlp_train_cfg* cfg, float* epoch_loss);
+/* Mean loss, its gradient (CPL_MLP_NPARAM floats) and optionally the FP32
+ * outputs (n x 8) of the training kernels over n records (gradient checks). */
+int cpl_mlp_grad(const float* features, const float* targets, size_t n, const float* params, float* grad_out,
+                 float* loss_out, float* outputs);
+/* Outputs 0..4 (n x 5) of the in-kernel BF16 MFMA inference path. */
+int cpl_mlp_infer(const float* features, size_t n, const float* params, float* out);
+/* Installs (params != NULL) or removes the proposal of a family for the
+ * CPL_OPT_LEARNED / CPL_OPT_LEARNED_SPLIT runs. */
+int cpl_mlp_install(int family, const float* params);
 
 #ifdef __cplusplus
 }

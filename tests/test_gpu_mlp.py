@@ -1,0 +1,139 @@
+"""Learned proposal on the GPU: training kernels against the float64 oracle
+(gradients, loss, outputs), the BF16 MFMA inference against its emulation,
+record collection against the run's own counts, training convergence, and
+learned runs with identical verdicts and less wasted FP32 work."""
+import numpy as np
+import pytest
+
+from circlepacking_amd import abi, learned
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _records(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 8)).astype(np.float32)
+    t = np.zeros((n, 8), dtype=np.float32)
+    t[:, 0] = rng.integers(0, 2, n)
+    t[:, 1:5] = rng.choice([0.0, 0.5, 1.0], (n, 4))
+    t[:, 5] = rng.integers(0, 2, n)
+    return x, t
+
+
+def test_training_kernels_match_oracle(cpl):
+    x, t = _records(1, 3000)
+    w = learned.init_params(7)
+    w[abi.MLP_B1:abi.MLP_W2] = 0.05
+    loss, g, y = cpl.mlp_grad(x, t, w, want_outputs=True)
+    ref_loss, ref_g = mlp_ref.loss_and_grad(x, t, w.astype(np.float64))
+    _, _, ref_y = mlp_ref.forward(x, w.astype(np.float64))
+    assert abs(loss - ref_loss) <= 1e-5 * abs(ref_loss)
+    assert np.max(np.abs(y - ref_y)) <= 1e-4 * max(1.0, np.max(np.abs(ref_y)))
+    err = np.abs(g - ref_g)
+    assert np.max(err) <= 1e-4 * np.max(np.abs(ref_g)), (np.argmax(err), np.max(err))
+    assert np.linalg.norm(g - ref_g) <= 1e-5 * np.linalg.norm(ref_g) * 10
+
+
+def test_mfma_inference_matches_bf16_emulation(cpl):
+    x, _ = _records(2, 1000)  # 15 full waves and a partial one
+    w = learned.init_params(3)
+    w[abi.MLP_B1:abi.MLP_W2] = np.linspace(-0.1, 0.1, 64)
+    w[abi.MLP_B2:abi.MLP_W3] = np.linspace(0.1, -0.1, 64)
+    w[abi.MLP_B3:] = np.arange(8) * 0.25
+    got = cpl.mlp_infer(x, w)
+    ref = mlp_ref.forward_bf16(x, w)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    assert np.max(np.abs(got - ref)) <= 2e-3 * scale, np.max(np.abs(got - ref))
+    # and close to the FP32 network (BF16 rounding only)
+    _, _, y = mlp_ref.forward(x, w.astype(np.float64))
+    assert np.max(np.abs(got - y[:, :5])) <= 5e-2 * scale
+
+
+def test_training_converges(cpl):
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (60_000, 8)).astype(np.float32)
+    t = np.zeros((60_000, 8), dtype=np.float32)
+    t[:, 0] = (x[:, 0] + 0.5 * x[:, 3] > 0.2)
+    t[:, 5] = 1.0
+    t[:, 1] = (x[:, 1] > 0) * 1.0
+    t[:, 2] = 0.5
+    w0 = learned.init_params(5)
+    w, losses = cpl.mlp_train(x, t, w0, epochs=6, batch=2048, lr=3e-3)
+    assert losses[-1] < 0.35 * losses[0], losses
+    acc = np.mean((cpl.mlp_infer(x, w)[:, 0] > 0) == (t[:, 0] > 0.5))
+    assert acc > 0.95
+
+
+def _small():
+    return abi.cfg_4param(grid=(16, 16, 16, 16))
+
+
+def test_collect_counts_match_the_run(cpl):
+    cfg = learned.family_cfg(_small(), abi.MIDDLE)
+    st, _ = cpl.refine_run(cfg)
+    feat, targ = cpl.mlp_collect(cfg, 5_000_000)
+    reach = sum(st.verdicts[d][v] for d in range(st.rounds) for v in (abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP))
+    assert feat.shape[0] == reach
+    assert np.all(np.abs(feat) <= 1.0)
+    v = targ[:, 6].astype(int)
+    assert set(np.unique(v)) <= {abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP}
+    assert np.all(targ[targ[:, 0] > 0, 6] == abi.V_PROVEN)  # FP32 never proves what FP64 does not
+    crit = (v == abi.V_CRITICAL) & (targ[:, 7] < cfg.max_depth)
+    assert np.array_equal(targ[:, 5] > 0, crit)
+    assert set(np.unique(targ[:, 1:5])) <= {0.0, 0.5, 1.0}
+    # subsampling keeps a deterministic subset
+    f2, _ = cpl.mlp_collect(cfg, 5_000_000, keep_every=4)
+    assert 0 < f2.shape[0] < feat.shape[0] / 2
+
+
+@pytest.fixture(scope="module")
+def trained(cpl):
+    cfg = abi.cfg_4param(grid=(24, 24, 24, 24))
+    models = learned.train(cpl, cfg, epochs=30, batch=1024)
+    yield models
+    for f in range(6):
+        cpl.mlp_install(f, None)
+
+
+@pytest.mark.parametrize("opts", [0, abi.OPT_GRAPH, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS])
+def test_learned_gate_identical_and_saves_fp32_work(cpl, trained, opts):
+    base = _small()
+    plain_s, plain_f = cpl.refine_run(base)
+    ungated, _ = cpl.refine_run(abi.with_options(base, opts | abi.OPT_FP32))
+    s, f = cpl.refine_run(abi.with_options(base, opts | abi.OPT_FP32 | abi.OPT_LEARNED, threshold=-1.0))
+    assert s.as_dict() == plain_s.as_dict()
+    assert np.array_equal(f, plain_f)
+    wasted_u = ungated.fp32_tried_total() - ungated.fp32_settled_total()
+    wasted_l = s.fp32_tried_total() - s.fp32_settled_total()
+    assert wasted_l < 0.5 * wasted_u, (wasted_l, wasted_u)
+    assert s.fp32_settled_total() > 0.7 * ungated.fp32_settled_total()
+
+
+def test_learned_split_runs_and_accounts(cpl, trained):
+    base = _small()
+    s, f = cpl.refine_run(abi.with_options(base, abi.OPT_LEARNED_SPLIT))
+    assert s.rounds >= 2 and s.total_evaluations > 0
+    leaves = s.certified + s.unresolved
+    splits = sum(s.verdicts[d][abi.V_CRITICAL] for d in range(s.rounds - 1))
+    assert s.total_evaluations == leaves + splits
+    assert np.all(f != 0)
+
+
+def test_learned_rejected_with_persistent_form(cpl, trained):
+    with pytest.raises(cpl.CplError):
+        cpl.refine_run(abi.with_options(_small(), abi.OPT_PERSISTENT | abi.OPT_HIER_ROOTS | abi.OPT_LEARNED))
+
+
+def test_learned_split_graph_matches_plain_learned_split(cpl, trained):
+    base = abi.with_options(_small(), abi.OPT_LEARNED_SPLIT)
+    a = cpl.refine_run(base)
+    b = cpl.refine_run(abi.with_options(base, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_LEARNED_SPLIT))
+    assert a[0].as_dict() == b[0].as_dict() and np.array_equal(a[1], b[1])
+
+
+def test_learned_requires_installed_proposal(cpl):
+    for fam in range(6):
+        cpl.mlp_install(fam, None)
+    with pytest.raises(cpl.CplError):
+        cpl.refine_run(abi.with_options(_small(), abi.OPT_FP32 | abi.OPT_LEARNED))
