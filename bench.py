@@ -5,10 +5,23 @@ One step = one complete synthetic 4-parameter proof run: lambda in
 64^4 root grid, families start / start&end 3-disk / middle / end 3-disk,
 adaptive widest-axis bisection of undecided cells to depth 6; every root cell
 is generated on the device, so the inputs are resident in HBM when timing
-starts.  Prints ONE JSON line (see README of the benchmark contract):
-value = (cell, family) leaves certified per second of the whole run.
+starts.  Prints ONE JSON line: value = (cell, family) leaves certified per
+second of the whole run in the fastest form whose verdicts, counts, digests and
+per-root flags are identical to the plain FP64 path.
 
-  python bench.py [--steps K] [--warmup W] [--variant plain] [--cpu-stride S] [--no-cpu]
+Forms timed (each checked for identical results against the plain path):
+  plain       one launch per family and round, host-synchronised sizes
+  graph       all rounds captured once as a hipGraph with device-side sizes,
+              hierarchical root generation, per-family parallel branches
+  persistent  one persistent kernel per family draining a mixed-depth stack
+  +fp32       the fastest batched form with the FP32 outward prefilter
+  +learned    graph + FP32 prefilter gated by the learned proposal (MLP
+              trained on the GPU from records of a 40^4 run of the same box)
+The learned split-axis policy changes the set of evaluated cells; it is timed
+and reported separately (evaluations, unresolved leaves), not as `value`.
+
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-tune] [--no-learned] [--forms plain,...]
+  python bench.py --traffic-probe   # 3 x 256 MiB device copy + one plain run (HBM counter calibration)
 """
 import argparse
 import json
@@ -23,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cells certified/s; time-to-proof, tuned vs plain FP64 GPU, identical verdicts"
 FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_round_traffic.json")
 
 
 def kernel_groups(stats, cfg):
@@ -38,18 +52,54 @@ def kernel_groups(stats, cfg):
     return groups
 
 
+def width_ratio_histogram(cpl, n=20000, seed=0):
+    """FP32 / FP64 width ratio of the criteria's geometry intervals on boxes of
+    the benchmark's domain (log2 bins)."""
+    rng = np.random.default_rng(seed)
+    la0 = rng.uniform(0.5, 0.98, n)
+    law = rng.choice([0.0, 1e-5, 1e-3], n)
+    lane = 1 - la0 - law
+    ra0 = lane * rng.uniform(0.25, 0.45, n)
+    rb0 = ra0 * rng.uniform(0.6, 1.0, n)
+    w = lane * rng.choice([1e-6, 1e-4, 1e-3, 1e-2], n)
+    recs = np.stack([la0, la0 + law, ra0, ra0 + w, rb0, rb0 + w * 0.5], 1)
+    ratios = []
+    for fn in ("inner_inner", "outer_inner", "inner_outer", "outer_outer", "semicircle_outer", "semicircle_inner"):
+        g64, t64 = cpl.debug_geom(fn, recs, fp32=False)
+        g32, t32 = cpl.debug_geom(fn, recs, fp32=True)
+        ok = (t64 == 0) & (t32 == 0) & (g64[:, 1] > g64[:, 0])
+        ratios.append((g32[ok, 1] - g32[ok, 0]) / (g64[ok, 1] - g64[ok, 0]))
+    r = np.concatenate(ratios)
+    edges = [1.0, 1.001, 1.01, 1.1, 2.0, 4.0, 16.0, np.inf]
+    hist, _ = np.histogram(r, bins=edges)
+    return {
+        "samples": int(r.size),
+        "bins": [f"[{a:g}, {b:g})" for a, b in zip(edges[:-1], edges[1:])],
+        "counts": [int(c) for c in hist],
+        "median": float(np.median(r)),
+        "p99": float(np.percentile(r, 99)),
+        "below_1": int(np.sum(r < 1.0)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default="tuned", choices=["plain", "tuned"],
-                    help="which form `value` reports; both are timed and checked for identical results")
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use 256 threads, 1 cell/thread)")
+                    help="which form `value` reports; every form is timed and checked for identical results")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (256 threads, 1 cell/thread)")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned proposal")
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--depth", type=int, default=6)
-    ap.add_argument("--cpu-stride", type=int, default=5, help="CPU baseline runs every S-th root cell")
+    ap.add_argument("--cpu-stride", type=int, default=2, help="CPU baseline runs every S-th root cell")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,graph,persistent,fp32",
+                    help="forms to time besides plain (graph, persistent, fp32); the learned forms follow --no-learned")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run a 256 MiB device copy three times and one plain run, then exit (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -57,46 +107,107 @@ def main():
         sys.exit(2)
 
     import circlepacking_amd as cpl
-    from circlepacking_amd import abi
+    from circlepacking_amd import abi, learned
 
+    if args.traffic_probe:
+        import torch
+
+        a = torch.ones(64 << 20, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        cpl.refine_run(abi.with_options(abi.cfg_4param(n=args.grid, max_depth=args.depth), abi.OPT_NO_DIGEST),
+                       want_flags=False)
+        print(json.dumps({"traffic_probe": True, "copy_bytes": 4 * a.numel()}), flush=True)
+        return
+    want = set(args.forms.split(","))
+
+    G, H, PER, F32, L, LS, ND = (abi.OPT_GRAPH, abi.OPT_HIER_ROOTS, abi.OPT_PERSISTENT, abi.OPT_FP32,
+                                 abi.OPT_LEARNED, abi.OPT_LEARNED_SPLIT, abi.OPT_NO_DIGEST)
     base = abi.cfg_4param(n=args.grid, max_depth=args.depth)
-    plain_cfg = abi.with_options(base, abi.OPT_NO_DIGEST)
-    tuned_cfg = abi.with_options(base, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST)
-    tune_table = None
-    if not args.no_tune:
-        tune_table = cpl.autotune(tuned_cfg, reps=3)
+    plain_ref = cpl.refine_run(abi.with_options(base, 0))  # digests + flags of the plain path
 
     def timed(cfg):
         for _ in range(args.warmup):
             cpl.refine_run(cfg, want_flags=False)
-        runs = []
         t0 = time.perf_counter()
+        st = None
         for _ in range(args.steps):
             st, _ = cpl.refine_run(cfg, want_flags=False)  # synchronous: returns after the device finished
-            runs.append(st)
         t1 = time.perf_counter()
-        return (t1 - t0) * 1e3 / args.steps, runs[-1]
+        return (t1 - t0) * 1e3 / args.steps, st
 
+    def identical(cfg):
+        s, f = cpl.refine_run(abi.with_options(cfg, cfg.options & ~ND))
+        return s.as_dict() == plain_ref[0].as_dict() and bool(np.array_equal(f, plain_ref[1]))
+
+    forms = {}
+    plain_cfg = abi.with_options(base, ND)
     ms_plain, st_plain = timed(plain_cfg)
-    ms_graph, _ = timed(tuned_cfg)
-    persist_cfg = abi.with_options(base, abi.OPT_PERSISTENT | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST,
-                                   block=tuned_cfg.block, cpt=1)
-    persist_table = None
-    if not args.no_tune:
-        persist_table = cpl.autotune(persist_cfg, reps=3)
-    ms_persist, _ = timed(persist_cfg)
-    if ms_persist < ms_graph:
-        tuned_cfg = persist_cfg
-    ms_tuned, st_tuned = timed(tuned_cfg)
-    # identical verdicts: full digests and per-root flags of both forms (untimed)
-    vp, fp = cpl.refine_run(abi.with_options(base, 0))
-    vt, ft = cpl.refine_run(abi.with_options(tuned_cfg, tuned_cfg.options & ~abi.OPT_NO_DIGEST))
-    identical = vp.as_dict() == vt.as_dict() and bool(np.array_equal(fp, ft))
+    forms["plain"] = {"ms": ms_plain, "identical": True}
+
+    graph_cfg = abi.with_options(base, G | H | ND)
+    persist_cfg = abi.with_options(base, PER | H | ND, block=64, cpt=1)
+    tune_graph = tune_persist = None
+    if not args.no_tune and "graph" in want:
+        tune_graph = cpl.autotune(graph_cfg, reps=3)
+    if not args.no_tune and "persistent" in want:
+        tune_persist = cpl.autotune(persist_cfg, reps=3)
+    for name, cfg in (("graph", graph_cfg), ("persistent", persist_cfg)):
+        if name not in want:
+            continue
+        ms, _ = timed(cfg)
+        forms[name] = {"ms": ms, "identical": identical(cfg), "block": cfg.block, "cells_per_thread": cfg.cells_per_thread}
+
+    batched = [k for k in ("graph", "persistent") if k in forms]
+    if batched and "fp32" in want:
+        # FP32 prefilter on the faster batched form
+        batched_name = min(batched, key=lambda k: forms[k]["ms"])
+        batched_cfg = graph_cfg if batched_name == "graph" else persist_cfg
+        f32_cfg = abi.with_options(batched_cfg, batched_cfg.options | F32)
+        ms, st = timed(f32_cfg)
+        criteria_evals = sum(st.verdicts[d][v] for d in range(st.rounds)
+                             for v in (abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP))
+        forms[batched_name + "+fp32"] = {
+            "ms": ms, "identical": identical(f32_cfg), "fp32_tried": st.fp32_tried_total(),
+            "fp32_settled": st.fp32_settled_total(),
+            "fraction_settled_in_fp32": st.fp32_settled_total() / max(1, criteria_evals),
+        }
+
+    learned_info = None
+    if not args.no_learned and "graph" in forms:
+        t0 = time.perf_counter()
+        train_cfg = abi.cfg_4param(grid=(40, 40, 40, 40), max_depth=args.depth)
+        models = learned.train(cpl, train_cfg, families=base.families(), epochs=12, batch=1024)
+        learned_info = {"train_seconds": time.perf_counter() - t0,
+                        "records": {abi.FAMILY_NAMES[f]: m[1]["records"] for f, m in models.items()},
+                        "final_loss": {abi.FAMILY_NAMES[f]: (m[1].get("loss") or [None])[-1] for f, m in models.items()}}
+        lg_cfg = abi.with_options(graph_cfg, G | H | ND | F32 | L, threshold=-0.5)
+        ms, st = timed(lg_cfg)
+        forms["graph+fp32+learned"] = {
+            "ms": ms, "identical": identical(lg_cfg), "fp32_tried": st.fp32_tried_total(),
+            "fp32_settled": st.fp32_settled_total(), "gate_threshold_logit": -0.5,
+        }
+        ls_cfg = abi.with_options(graph_cfg, G | H | ND | LS)
+        ms, st = timed(ls_cfg)
+        learned_info["learned_split"] = {
+            "ms": ms, "evaluations": st.total_evaluations, "certified": st.certified, "unresolved": st.unresolved,
+            "plain_evaluations": st_plain.total_evaluations, "plain_unresolved": st_plain.unresolved,
+            "note": "changes the set of evaluated cells (another bisection order); not an identical-verdict form",
+        }
+        for f in range(6):
+            cpl.mlp_install(f, None)
+
+    # value: the fastest identical form
+    ok = {k: v for k, v in forms.items() if v["identical"]}
+    tuned_name = min(ok, key=lambda k: ok[k]["ms"])
     if args.variant == "plain":
-        ms_per_step, st = ms_plain, st_plain
+        name = "plain"
     else:
-        ms_per_step, st = ms_tuned, st_tuned
-    certified = st.certified
+        name = tuned_name
+    ms_per_step = forms[name]["ms"]
+    certified = st_plain.certified
     value = certified / (ms_per_step * 1e-3)
 
     # ---- CPU oracle on a stride sample: baseline timing + per-evaluation flop counts ----
@@ -105,7 +216,7 @@ def main():
     if not args.no_cpu:
         from oracle import cplo
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         c0 = time.perf_counter()
         ost, _ = cplo.refine_run(base, want_flags=False, root_stride=args.cpu_stride, nthreads=threads)
         c1 = time.perf_counter()
@@ -114,7 +225,7 @@ def main():
             "unit": "cells/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"every {args.cpu_stride}th root cell of the same 64^4 grid ({ost.total_evaluations} "
+            "sample": f"every {args.cpu_stride}th root cell of the same {args.grid}^4 grid ({ost.total_evaluations} "
                       f"evaluations, {ost.certified} certified leaves, {c1 - c0:.1f} s wall)",
         }
         fst, _, flops = cplo.refine_run(base, want_flags=False, root_stride=max(args.cpu_stride, 97),
@@ -125,26 +236,29 @@ def main():
         with np.errstate(invalid="ignore", divide="ignore"):
             flops_per_eval = np.where(counts > 0, flops / np.maximum(counts, 1), 0.0)
 
-    # ---- roofline of the dominant kernel (largest total device time) ----
-    cfg = plain_cfg
-    groups = kernel_groups(st_plain, cfg)
+    # ---- roofline of the dominant kernel of the plain path (largest total device time) ----
+    groups = kernel_groups(st_plain, plain_cfg)
     roof = None
     if groups:
         key = max(groups, key=lambda k: sum(s for _, s, _ in groups[k]))
         launches = groups[key]
         avg_sec = sum(s for _, s, _ in launches) / len(launches)
         f, is_root = key
-        name = f"k_round<{f}, {'true' if is_root else 'false'}>"
+        kname = f"k_round<{f}, {'true' if is_root else 'false'}, 256, false>"
         achieved = None
         if flops_per_eval is not None:
             flops_per_launch = sum(float(np.dot(c, flops_per_eval[f])) for _, _, c in launches) / len(launches)
             achieved = flops_per_launch / avg_sec / 1e12
-        # worklist bytes: every item read (64 B endpoints + 12 B keys), two children written per split
         items = sum(float(c.sum()) for _, _, c in launches) / len(launches)
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as fh:
+                t = json.load(fh)
+            traffic = t.get(kname)
         roof = {
             "bound": "mfma",
             "pipe": "FP64 VALU (directed-rounding interval arithmetic; published FP64 peak)",
-            "kernel": name,
+            "kernel": kname,
             "family": abi.FAMILY_NAMES[f],
             "launches": len(launches),
             "avg_launch_ms": avg_sec * 1e3,
@@ -152,7 +266,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved is not None else None,
-            "traffic": None,
+            "traffic": traffic,
             "worklist_items_per_launch": items,
         }
 
@@ -175,25 +289,21 @@ def main():
             "grid": [args.grid] * 4,
             "max_depth": args.depth,
             "variant": args.variant,
-            "tuned_options": ("persistent mixed-depth queue" if tuned_cfg.options & abi.OPT_PERSISTENT
-                              else "hipGraph rounds") + " + hierarchical roots" if args.variant == "tuned" else None,
-            "block": tuned_cfg.block if args.variant == "tuned" else 256,
-            "cells_per_thread": tuned_cfg.cells_per_thread if args.variant == "tuned" else 1,
+            "form": name,
         },
-        "plain_ms_per_step": ms_plain,
-        "tuned_ms_per_step": ms_tuned,
-        "graph_ms_per_step": ms_graph,
-        "persistent_ms_per_step": ms_persist,
-        "tuned_speedup": ms_plain / ms_tuned,
-        "identical_verdicts": identical,
-        "autotune": tune_table,
-        "autotune_persistent": persist_table,
+        "forms": forms,
+        "tuned_form": tuned_name,
+        "tuned_speedup": forms["plain"]["ms"] / forms[tuned_name]["ms"],
+        "identical_verdicts": all(v["identical"] for v in forms.values()),
+        "autotune_graph": tune_graph,
+        "autotune_persistent": tune_persist,
+        "learned": learned_info,
+        "fp32_width_ratio": width_ratio_histogram(cpl) if "fp32" in want else None,
         "time_to_proof_ms": ms_per_step,
-        "device_ms_per_step": st.seconds * 1e3,
-        "evaluations": st.total_evaluations,
-        "evaluations_per_s": st.total_evaluations / (ms_per_step * 1e-3),
+        "evaluations": st_plain.total_evaluations,
+        "evaluations_per_s": st_plain.total_evaluations / (ms_per_step * 1e-3),
         "certified": certified,
-        "unresolved": st.unresolved,
+        "unresolved": st_plain.unresolved,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

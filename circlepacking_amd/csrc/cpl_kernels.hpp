@@ -3,7 +3,7 @@
 
 #include "cpl_lane.hpp"
 #include "cpl_mlp.hpp"
-#include "../../include/cpl.h"
+#include "cpl.h"
 
 namespace cpl {
 

@@ -48,6 +48,15 @@ __device__ __forceinline__ IVT<T> asin_iv(IVT<T> x, int& trap) {
   return IVT<T>(dasin_rd(x.lb), dasin_ru(x.ub));
 }
 
+// Result of an out-of-line geometry function: the angle interval and the trap
+// code (0: none), returned in registers (a trap flag passed by reference would
+// live in memory).
+template <class T>
+struct Angle {
+  IVT<T> v;
+  int trap;
+};
+
 // collision_angles.cu:29-31
 template <class T>
 __device__ __forceinline__ UB is_obtuse(IVT<T> hyp, IVT<T> s1, IVT<T> s2) {
@@ -56,7 +65,8 @@ __device__ __forceinline__ UB is_obtuse(IVT<T> hyp, IVT<T> s1, IVT<T> s2) {
 
 // collision_angles.cu:37-117
 template <class T>
-__device__ __attribute__((noinline)) IVT<T> compute_collision_angle(IVT<T> lc, IVT<T> sc, IVT<T> ab, int& trap) {
+__device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> lc, IVT<T> sc, IVT<T> ab) {
+  int trap = 0;
   using I = IVT<T>;
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
@@ -74,19 +84,19 @@ __device__ __attribute__((noinline)) IVT<T> compute_collision_angle(IVT<T> lc, I
     sx.tighten_ub(sc.ub);
     lx.tighten_ub(lc.ub);
     if (sx.empty() || lx.empty()) {
-      if (definitely(obt)) { trap = T_OBT_X; return result; }
+      if (definitely(obt)) { trap = T_OBT_X; return Angle<T>{result, trap}; }
     } else {
       I lquot = lx / lc;
       lquot.tighten_ub(T(1));
       I squot = sx / sc;
       squot.tighten_ub(T(1));
       if (lquot.empty() || squot.empty()) {
-        if (definitely(obt)) { trap = T_OBT_Q; return result; }
+        if (definitely(obt)) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
       } else {
         I alpha = asin_iv(lquot, trap);
-        if (trap) return result;
+        if (trap) return Angle<T>{result, trap};
         I beta = asin_iv(squot, trap);
-        if (trap) return result;
+        if (trap) return Angle<T>{result, trap};
         result = alpha - beta;
         result.tighten_lb(T(0));
       }
@@ -98,24 +108,31 @@ __device__ __attribute__((noinline)) IVT<T> compute_collision_angle(IVT<T> lc, I
     I sx = T(0.5) * (ab - add_term);
     sx.tighten_lb(T(0));
     if (sx.empty()) {
-      if (!possibly(obt)) { trap = T_ACU_X; return result; }
+      if (!possibly(obt)) { trap = T_ACU_X; return Angle<T>{result, trap}; }
     } else {
       I alpha = asin_iv(lx / lc, trap);
-      if (trap) return result;
+      if (trap) return Angle<T>{result, trap};
       I beta = asin_iv(sx / sc, trap);
-      if (trap) return result;
+      if (trap) return Angle<T>{result, trap};
       beta.tighten_lb(T(0));
       I r = alpha + beta;
       if (result.empty()) result = r;
       else result.do_join(r);
     }
   }
-  return result;
+  return Angle<T>{result, trap};
+}
+template <class T>
+__device__ __forceinline__ IVT<T> compute_collision_angle(IVT<T> lc, IVT<T> sc, IVT<T> ab, int& trap) {
+  const Angle<T> a = compute_collision_angle_r(lc, sc, ab);
+  if (a.trap) trap = a.trap;
+  return a.v;
 }
 
 // collision_angles.cu:119-142
 template <class T>
-__device__ __attribute__((noinline)) IVT<T> collision_angle_inner_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+__device__ __attribute__((noinline)) Angle<T> collision_angle_inner_inner_r(IVT<T> la, IVT<T> ra, IVT<T> rb) {
+  int trap = 0;
   using I = IVT<T>;
   T lb, ub;
   {
@@ -123,7 +140,7 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_inner_inner(IVT<T> l
     I sc = la.lb + I(rb.ub, rb.ub);
     I ab = I(ra.ub, ra.ub) + rb.ub;
     ub = compute_collision_angle(lc, sc, ab, trap).ub;
-    if (trap) return I(T(0), T(0));
+    if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   {
     I lc = la.ub + I(ra.lb, ra.lb);
@@ -132,7 +149,13 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_inner_inner(IVT<T> l
     if (ab.lb <= T(0)) lb = T(0);
     else lb = compute_collision_angle(lc, sc, ab, trap).lb;
   }
-  return I(lb, ub);
+  return Angle<T>{I(lb, ub), trap};
+}
+template <class T>
+__device__ __forceinline__ IVT<T> collision_angle_inner_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  const Angle<T> a = collision_angle_inner_inner_r(la, ra, rb);
+  if (a.trap) trap = a.trap;
+  return a.v;
 }
 
 // collision_angles.cu:144-159 (outer_inner: ca = 1 - ra, cb = la + rb) and
@@ -150,7 +173,8 @@ __device__ __forceinline__ IVT<T> do_collision_angle(IVT<T> ca, IVT<T> cb, IVT<T
 
 // collision_angles.cu:161-174
 template <class T>
-__device__ __attribute__((noinline)) IVT<T> collision_angle_outer_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+__device__ __attribute__((noinline)) Angle<T> collision_angle_outer_inner_r(IVT<T> la, IVT<T> ra, IVT<T> rb) {
+  int trap = 0;
   using I = IVT<T>;
   T lb, ub;
   if (possibly(T(2) * (ra + rb) <= T(1) - la)) {
@@ -158,16 +182,23 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_outer_inner(IVT<T> l
   } else {
     I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
     lb = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).lb;
-    if (trap) return I(T(0), T(0));
+    if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
   ub = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).ub;
-  return I(lb, ub);
+  return Angle<T>{I(lb, ub), trap};
+}
+template <class T>
+__device__ __forceinline__ IVT<T> collision_angle_outer_inner(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  const Angle<T> a = collision_angle_outer_inner_r(la, ra, rb);
+  if (a.trap) trap = a.trap;
+  return a.v;
 }
 
 // collision_angles.cu:193-205
 template <class T>
-__device__ __attribute__((noinline)) IVT<T> collision_angle_inner_outer(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+__device__ __attribute__((noinline)) Angle<T> collision_angle_inner_outer_r(IVT<T> la, IVT<T> ra, IVT<T> rb) {
+  int trap = 0;
   using I = IVT<T>;
   T lb, ub;
   if (possibly(T(2) * (ra + rb) <= T(1) - la)) {
@@ -175,16 +206,23 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_inner_outer(IVT<T> l
   } else {
     I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
     lb = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).lb;
-    if (trap) return I(T(0), T(0));
+    if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
   ub = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).ub;
-  return I(lb, ub);
+  return Angle<T>{I(lb, ub), trap};
+}
+template <class T>
+__device__ __forceinline__ IVT<T> collision_angle_inner_outer(IVT<T> la, IVT<T> ra, IVT<T> rb, int& trap) {
+  const Angle<T> a = collision_angle_inner_outer_r(la, ra, rb);
+  if (a.trap) trap = a.trap;
+  return a.v;
 }
 
 // collision_angles.cu:207-231
 template <class T>
-__device__ __attribute__((noinline)) IVT<T> collision_angle_outer_outer(IVT<T> ra, IVT<T> rb, int& trap) {
+__device__ __attribute__((noinline)) Angle<T> collision_angle_outer_outer_r(IVT<T> ra, IVT<T> rb) {
+  int trap = 0;
   using I = IVT<T>;
   T lb, ub;
   {
@@ -192,7 +230,7 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_outer_outer(IVT<T> r
     I sc = I(T(1), T(1)) - ra.ub;
     I ab = I(ra.ub, ra.ub) + rb.ub;
     ub = compute_collision_angle(lc, sc, ab, trap).ub;
-    if (trap) return I(T(0), T(0));
+    if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   {
     I lc = I(T(1), T(1)) - rb.lb;
@@ -201,7 +239,13 @@ __device__ __attribute__((noinline)) IVT<T> collision_angle_outer_outer(IVT<T> r
     if (ab.lb <= T(0)) lb = T(0);
     else lb = compute_collision_angle(lc, sc, ab, trap).lb;
   }
-  return I(lb, ub);
+  return Angle<T>{I(lb, ub), trap};
+}
+template <class T>
+__device__ __forceinline__ IVT<T> collision_angle_outer_outer(IVT<T> ra, IVT<T> rb, int& trap) {
+  const Angle<T> a = collision_angle_outer_outer_r(ra, rb);
+  if (a.trap) trap = a.trap;
+  return a.v;
 }
 
 // collision_angles.cu:233-238

@@ -134,7 +134,7 @@ CPL_SCALE_DIR(scale_ru, CPL_MODE_RU)
 // direction onto nested grids equals rounding once, so underflowing and
 // overflowing quotients come out exactly as IEEE directed division gives them.
 template <bool UP>
-__device__ __attribute__((noinline)) double div_slow(double a, double b) {
+__device__ __forceinline__ double div_slow(double a, double b) {
   double q = a / b;
   if (__builtin_isnan(q) || a == 0.0 || __builtin_isinf(a) || b == 0.0 || __builtin_isinf(b)) return q;
   if (b < 0.0) { a = -a; b = -b; }
