@@ -10,6 +10,9 @@
 //   --subdiv a,b,c,d   subdivision counts instead of the reference's
 //   --lambda lo,hi     outer lambda interval instead of [lambda_min, lambda_max]
 //   --max-launches N   stop a stage after N launches (reports it and exits 3)
+//   --checkpoint FILE  after every launch, save the stage and its lambda queue
+//   --resume FILE      continue a run from a saved queue (earlier stages are
+//                      skipped; the saved stage resumes with its queue)
 #include <cfloat>
 #include <cstdio>
 #include <cstdlib>
@@ -19,7 +22,7 @@
 #include <system_error>
 #include <vector>
 
-#include "../../include/cpl.h"
+#include "cpl.h"
 #include "run_until_no_criticals.hpp"
 
 namespace {
@@ -54,6 +57,10 @@ struct Options {
   int32_t subdiv[4] = {0, 0, 0, 0};
   double lo = 0.5, hi = 0.9900000000000001;  // values.hpp:31-32
   long max_launches = -1;
+  std::string checkpoint;
+  bool resume = false;
+  int resume_family = -1;
+  std::vector<circlepacking::IV> resume_queue;
 };
 Options g_opt;
 
@@ -79,10 +86,22 @@ struct HipLauncher {
 
 bool run_stage(int family) {
   HipLauncher launch{family};
-  return circlepacking::run_kernel_until_no_criticals(launch, circlepacking::IV(g_opt.lo, g_opt.hi));
+  circlepacking::QueueCheckpoint cp;
+  const std::string tag = "stage " + std::to_string(family);
+  if (!g_opt.checkpoint.empty())
+    cp.save = [&](const std::vector<circlepacking::IV>& q) {
+      if (!circlepacking::write_queue_file(g_opt.checkpoint, tag, q))
+        std::cerr << "cannot write checkpoint " << g_opt.checkpoint << std::endl;
+    };
+  if (g_opt.resume && family == g_opt.resume_family) {
+    cp.resume = true;
+    cp.resume_from = g_opt.resume_queue;
+  }
+  return circlepacking::run_kernel_until_no_criticals(launch, circlepacking::IV(g_opt.lo, g_opt.hi), std::cerr, &cp);
 }
 
-bool want(int f) { return g_opt.family < 0 || g_opt.family == f; }
+// Stage selection: --family, and on resume every stage before the saved one is done.
+bool want(int f) { return (g_opt.family < 0 || g_opt.family == f) && (!g_opt.resume || f >= g_opt.resume_family); }
 
 }  // namespace
 
@@ -154,8 +173,22 @@ int main(int argc, char** argv) {
       if (std::sscanf(next(), "%lf,%lf", &g_opt.lo, &g_opt.hi) != 2) return 2;
     } else if (a == "--max-launches") {
       g_opt.max_launches = std::atol(next());
+    } else if (a == "--checkpoint") {
+      g_opt.checkpoint = next();
+    } else if (a == "--resume") {
+      std::string tag;
+      const char* path = next();
+      if (!circlepacking::read_queue_file(path, tag, g_opt.resume_queue) ||
+          std::sscanf(tag.c_str(), "stage %d", &g_opt.resume_family) != 1) {
+        std::fprintf(stderr, "cannot read checkpoint %s\n", path);
+        return 2;
+      }
+      g_opt.resume = true;
     } else {
-      std::fprintf(stderr, "usage: %s [--family F] [--subdiv a,b,c,d] [--lambda lo,hi] [--max-launches N]\n", argv[0]);
+      std::fprintf(stderr,
+                   "usage: %s [--family F] [--subdiv a,b,c,d] [--lambda lo,hi] [--max-launches N] "
+                   "[--checkpoint FILE] [--resume FILE]\n",
+                   argv[0]);
       return 2;
     }
   }

@@ -71,3 +71,46 @@ def test_sigint_reruns_first_remaining_interval_with_output(trace_bin):
     interrupted = launches[k]
     if interrupted[2] != float("inf"):
         assert last[:2] == interrupted[:2]
+
+
+@pytest.fixture(scope="module")
+def ckpt_bin(tmp_path_factory):
+    out = tmp_path_factory.mktemp("drv") / "driver_checkpoint"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(out), os.path.join(ROOT, "tests", "driver_checkpoint.cpp")],
+                   check=True)
+    return str(out)
+
+
+def _launches(stdout):
+    return [l for l in stdout.strip().splitlines() if not l.startswith("result")]
+
+
+@pytest.mark.parametrize("stop", [1, 7, 15])
+def test_checkpoint_resume_reproduces_the_uninterrupted_trace(ckpt_bin, tmp_path, stop):
+    full = subprocess.run([ckpt_bin, "run", str(tmp_path / "full.q"), "-1"], capture_output=True, text=True, check=True)
+    path = str(tmp_path / "part.q")
+    a = subprocess.run([ckpt_bin, "run", path, str(stop)], capture_output=True, text=True)
+    assert a.returncode == 3
+    b = subprocess.run([ckpt_bin, "resume", path], capture_output=True, text=True, check=True)
+    assert _launches(a.stdout) + _launches(b.stdout) == _launches(full.stdout)
+    assert b.stdout.strip().splitlines()[-1] == "result 1"
+    # the finished run leaves an empty queue; resuming it launches nothing
+    c = subprocess.run([ckpt_bin, "resume", path], capture_output=True, text=True, check=True)
+    assert c.stdout.strip() == "result 1"
+
+
+def test_interval_text_roundtrip_is_exact(ckpt_bin):
+    import numpy as np
+
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([rng.uniform(-1, 1, 2000), rng.standard_normal(500) * 1e300, [5e-324, -0.0, 0.1, 1 / 3]])
+    vals = vals[np.isfinite(vals)]
+    lo, hi = np.minimum(vals, vals[::-1]), np.maximum(vals, vals[::-1])
+    text = "\n".join(f"[{float(a)!r}, {float(b)!r}]" for a, b in zip(lo, hi)) + "\n[0.25; 0.5]\n0.75\n"
+    p = subprocess.run([ckpt_bin, "roundtrip"], input=text, capture_output=True, text=True, check=True)
+    out = p.stdout.strip().splitlines()
+    assert len(out) == len(lo) + 2
+    for (a, b), line in zip(zip(lo, hi), out):
+        x, y = line.strip()[1:-1].split(",")
+        assert float(x) == a and float(y) == b
+    assert out[-2].replace(" ", "") == "[0.25,0.5]" and out[-1].replace(" ", "") == "[0.75,0.75]"

@@ -58,3 +58,25 @@ def test_prover_banners_in_reference_order():
         assert all(p >= 0 for p in pos) and pos == sorted(pos)
     else:
         assert g.returncode == 3 and pos[0] >= 0
+
+
+def test_checkpoint_resume_continues_the_trace(tmp_path):
+    """A run stopped after 5 launches and resumed from its checkpoint handles
+    the same lambda intervals, in the same order, as an uninterrupted run."""
+    fam, subdiv, lam = abi.MIDDLE, "6,6,6,3", "0.5,0.56"
+    full = subprocess.run([PROVER, "--family", str(fam), "--subdiv", subdiv, "--lambda", lam, "--max-launches", "30"],
+                          capture_output=True, text=True, timeout=120)
+    ck = str(tmp_path / "queue.txt")
+    a = subprocess.run([PROVER, "--family", str(fam), "--subdiv", subdiv, "--lambda", lam, "--max-launches", "5",
+                        "--checkpoint", ck], capture_output=True, text=True, timeout=120)
+    ha = handled(a.stderr)
+    if a.returncode == 0:  # the whole stage took at most 5 launches
+        assert ha == handled(full.stderr)
+        return
+    assert a.returncode == 3 and len(ha) == 6  # the 6th interval was popped, then the limit stopped the launch
+    b = subprocess.run([PROVER, "--family", str(fam), "--subdiv", subdiv, "--lambda", lam, "--max-launches", "25",
+                        "--resume", ck, "--checkpoint", ck], capture_output=True, text=True, timeout=120)
+    assert b.returncode == full.returncode
+    hf = handled(full.stderr)
+    assert ha[:5] + handled(b.stderr) == hf[:5 + len(handled(b.stderr))]
+    assert len(ha[:5] + handled(b.stderr)) == len(hf)
