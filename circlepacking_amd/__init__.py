@@ -39,6 +39,7 @@ def lib():
         L.cpl_debug_geom.argtypes = [ctypes.c_int, ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int),
                                      ctypes.c_size_t]
         L.cpl_debug_prefilter.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8, _P_U8]
+        L.cpl_interval_op.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
         _P_F = ctypes.POINTER(ctypes.c_float)
         L.cpl_mlp_collect.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.c_uint64, ctypes.c_uint32, _P_F, _P_F,
                                       ctypes.POINTER(ctypes.c_uint64)]
@@ -69,7 +70,7 @@ def prim(op, a, b=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     out = np.empty_like(a)
     bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
-    code = abi.PRIM_OPS[op] if op in abi.PRIM_OPS else abi.PRIM32_OPS[op]
+    code = abi.prim_code(op)
     _check(lib().cpl_debug_prim(code, _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size))
     return out
 
@@ -87,6 +88,19 @@ def debug_geom(fn, recs, fp32=False):
     trap = np.zeros(n, dtype=np.int32)
     _check(lib().cpl_debug_geom(GEOM[fn], int(bool(fp32)), _dptr(recs), _dptr(out),
                                 trap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
+    return out, trap
+
+
+def interval_op(op, a, b=None):
+    """Elementwise interval operation (name from abi.IV_OPS) on (n, 2) arrays of
+    [lb, ub]; returns ((n, 2) result, (n,) trap codes)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 2)
+    n = a.shape[0]
+    bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
+    out = np.zeros((n, 2), dtype=np.float64)
+    trap = np.zeros(n, dtype=np.int32)
+    _check(lib().cpl_interval_op(abi.IV_OPS[op], _dptr(a), None if bb is None else _dptr(bb), _dptr(out),
+                                 trap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n))
     return out, trap
 
 

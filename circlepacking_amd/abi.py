@@ -41,6 +41,24 @@ PRIM_OPS = {
     "div_rd": 6, "div_ru": 7, "sqrt_rd": 8, "sqrt_ru": 9, "asin_rn": 10,
     "dasin_rd": 11, "dasin_ru": 12, "lbrsp": 13, "div_rn": 14, "sqrt_rn": 15,
 }
+PRIM_TRIG_OPS = {
+    "sin_rn": 29, "cos_rn": 30, "acos_rn": 31, "dsin_rd": 32, "dsin_ru": 33, "dcos_rd": 34, "dcos_ru": 35,
+    "dacos_rd": 36, "dacos_ru": 37,
+}
+IV_OPS = {
+    "add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "abs": 6, "reciprocal": 7,
+    "div_scalar": 8, "scalar_div": 9, "sin": 10, "cos": 11, "asin": 12, "acos": 13, "mul_scalar": 14, "neg": 15,
+}
+IV_BINARY = {"add", "sub", "mul", "div", "div_scalar", "scalar_div", "mul_scalar"}
+
+
+def prim_code(op):
+    for table in (PRIM_OPS, PRIM32_OPS, PRIM_TRIG_OPS):
+        if op in table:
+            return table[op]
+    raise KeyError(op)
+
+
 # FP32 prefilter primitives (operands and results are float values carried in doubles)
 PRIM32_OPS = {
     "add_rd32": 16, "add_ru32": 17, "sub_rd32": 18, "sub_ru32": 19, "mul_rd32": 20, "mul_ru32": 21,

@@ -28,6 +28,7 @@
 
 #include "cpl_host.hpp"
 #include "cpl_kernels.hpp"
+#include "cpl_trig.hpp"
 
 using namespace cpl;
 
@@ -69,9 +70,48 @@ __global__ void k_debug_prim(int op, const double* a, const double* b, double* o
     case 26: r = dasin_ru((float)x); break;
     case 27: r = f32_down(x); break;
     case 28: r = f32_up(x); break;
+    case 29: r = sin_rn(x); break;
+    case 30: r = cos_rn(x); break;
+    case 31: r = acos_rn(x); break;
+    case 32: r = dsin_rd(x); break;
+    case 33: r = dsin_ru(x); break;
+    case 34: r = dcos_rd(x); break;
+    case 35: r = dcos_ru(x); break;
+    case 36: r = dacos_rd(x); break;
+    case 37: r = dacos_ru(x); break;
     default: r = __builtin_nan("");
   }
   out[i] = r;
+}
+
+// Elementwise interval operations (the algcuda::Interval<double> surface).
+__global__ void k_interval_op(int op, const IV* a, const IV* b, IV* out, int* trap, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const IV x = a[i];
+  const IV y = b ? b[i] : IV(0.0, 0.0);
+  IV r(0.0, 0.0);
+  int t = 0;
+  switch (op) {
+    case 0: r = x + y; break;
+    case 1: r = x - y; break;
+    case 2: r = x * y; break;
+    case 3: r = x / y; break;
+    case 4: r = sqrt_iv(x, t); break;
+    case 5: r = x.square(); break;
+    case 6: r = x.abs(); break;
+    case 7: r = reciprocal_iv(x); break;
+    case 8: r = div_scalar(x, y.lb); break;
+    case 9: r = IV(x.lb, x.lb) / y; break;
+    case 10: r = sin_iv(x); break;
+    case 11: r = cos_iv(x); break;
+    case 12: r = asin_iv(x, t); break;
+    case 13: r = acos_iv(x, t); break;
+    case 14: r = x * y.lb; break;
+    default: r = neg(x); break;
+  }
+  out[i] = r;
+  trap[i] = t;
 }
 
 // Geometry functions of the criteria on interval records (la, ra, rb), in
@@ -1073,7 +1113,7 @@ int cpl_shutdown(void) {
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (op < 0 || op > 28 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
+  if (op < 0 || op > 37 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
   if (n == 0) return CPL_OK;
   size_t bytes = n * sizeof(double);
   if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)))
@@ -1140,6 +1180,29 @@ int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, s
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, g.scratch_out.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
   HIP_TRY(hipMemcpyAsync(trap, g.scratch_in2.p, n * sizeof(int), hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return CPL_OK;
+}
+
+int cpl_interval_op(int op, const cpl_iv* a, const cpl_iv* b, cpl_iv* out, int* trap, size_t n) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (op < 0 || op > 15 || !a || !out || !trap) return fail(CPL_ERR_ARG, "cpl_interval_op: bad op or pointer");
+  if (op <= 3 || op == 8 || op == 9 || op == 14)
+    if (!b) return fail(CPL_ERR_ARG, "cpl_interval_op: binary operation needs b");
+  if (n == 0) return CPL_OK;
+  const size_t bytes = n * sizeof(cpl_iv);
+  if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)) ||
+      (rc = g.verdicts.ensure(n * sizeof(int))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(g.scratch_in.p, a, bytes, hipMemcpyHostToDevice, g.stream));
+  if (b) HIP_TRY(hipMemcpyAsync(g.scratch_in2.p, b, bytes, hipMemcpyHostToDevice, g.stream));
+  hipLaunchKernelGGL(k_interval_op, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g.stream, op,
+                     g.scratch_in.as<IV>(), b ? g.scratch_in2.as<IV>() : nullptr, g.scratch_out.as<IV>(),
+                     g.verdicts.as<int>(), n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, g.scratch_out.p, bytes, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipMemcpyAsync(trap, g.verdicts.p, n * sizeof(int), hipMemcpyDeviceToHost, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
   return CPL_OK;
 }

@@ -207,7 +207,27 @@ int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t
 /* FP32 prefilter primitives through the same entry point (operands and
  * results are float values carried in doubles): 16 add_rd 17 add_ru 18 sub_rd
  * 19 sub_ru 20 mul_rd 21 mul_ru 22 div_rd 23 div_ru 24 asin_rn 25 dasin_rd
- * 26 dasin_ru; 27 / 28: a double rounded down / up to float. */
+ * 26 dasin_ru; 27 / 28: a double rounded down / up to float.  Round-to-nearest
+ * and widened functions of the interval surface (argument in [0, 8] for sin and
+ * cos): 29 sin_rn 30 cos_rn 31 acos_rn 32 dsin_rd 33 dsin_ru 34 dcos_rd
+ * 35 dcos_ru 36 dacos_rd 37 dacos_ru. */
+
+/* Elementwise interval operations of the algcuda::Interval<double> surface
+ * (interval.cuh) on n intervals; b is the second operand (binary ops; for the
+ * scalar forms its lb is the scalar).  trap: n codes (CPL_TRAP_*; 0 none).
+ * Differences to the reference, both in the enclosing direction: scalar x
+ * interval rounds the upper bound up (interval.cuh:185-197 rounds it down) and
+ * the reciprocal of a negative interval is [rd(1/ub), ru(1/lb)]
+ * (interval.cuh:227-235 returns the inverted pair).  sin / cos / acos use the
+ * library's own round-to-nearest functions widened by 3 ulps. */
+enum {
+  CPL_IV_ADD = 0, CPL_IV_SUB = 1, CPL_IV_MUL = 2, CPL_IV_DIV = 3,   /* interval.cuh:38-66, 171-182, 277-316 */
+  CPL_IV_SQRT = 4, CPL_IV_SQUARE = 5, CPL_IV_ABS = 6,               /* :330-355, interval.hpp:216-232 */
+  CPL_IV_RECIPROCAL = 7, CPL_IV_DIV_SCALAR = 8, CPL_IV_SCALAR_DIV = 9, /* :227-235, 261-275 */
+  CPL_IV_SIN = 10, CPL_IV_COS = 11, CPL_IV_ASIN = 12, CPL_IV_ACOS = 13, /* :578-624 */
+  CPL_IV_MUL_SCALAR = 14, CPL_IV_NEG = 15                             /* :185-197, interval.hpp:131 */
+};
+int cpl_interval_op(int op, const cpl_iv* a, const cpl_iv* b, cpl_iv* out, int* trap, size_t n);
 
 /* Geometry of the criteria on n records of 6 doubles (la, ra, rb intervals):
  * fn 0 collision_angle_inner_inner 1 outer_inner 2 inner_outer (collision_angles.cu:119-205)

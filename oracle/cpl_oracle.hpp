@@ -331,6 +331,170 @@ static inline IV asin(const IV& x) {
 }
 
 // ---------------------------------------------------------------------------
+// Rest of the interval surface (interval.cuh:217-235, 357-418, 440-615): not
+// on the lane-packing path.  CUDA's ::sin, ::cos, ::acos are closed source;
+// sin_rn, cos_rn, acos_rn are the project's own round-to-nearest functions
+// (three-part Cody-Waite reduction by pi/2 and Chebyshev kernels from
+// circlepacking_amd/csrc/gen_trig_poly.py; acos from the arcsine kernel), the
+// same operation sequence as the HIP product.  Directed versions step SC_STEPS
+// ulps outward as the reference does.
+// ---------------------------------------------------------------------------
+static const int SC_STEPS = 3;
+static const double lb_3pi_two = 4.71238898038469, ub_3pi_two = 4.712388980384691;
+static const double lb_two_pi = 6.283185307179586, ub_two_pi = 6.283185307179587;
+static const double trig_p1 = 0x1.921fb54442d18p+0, trig_p2 = 0x1.1a62633145c07p-54, trig_p3 = -0x1.f1976b7ed8fbcp-110;
+static const double two_over_pi = 0x1.45f306dc9c883p-1;
+static const double sin_coef[8] = {0x1.9242bc8544292p-49, -0x1.ae7c53ccfc679p-41, 0x1.61245fb507691p-33,
+                                   -0x1.ae64567e73219p-26, 0x1.71de3a556c2c8p-19, -0x1.a01a01a01a019p-13,
+                                   0x1.1111111111111p-7,   -0x1.5555555555555p-3};
+static const double cos_coef[8] = {-0x1.65d2c0a3f787dp-53, 0x1.ae7ce8e10b0f8p-45, -0x1.93974954490b6p-37,
+                                   0x1.1eed8eff336c1p-29,  -0x1.27e4fb7789d98p-22, 0x1.a01a01a01a019p-16,
+                                   -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5};
+
+static inline double trig_poly(const double* c, double z) {
+  double p = c[0];
+  for (int i = 1; i < 8; ++i) p = std::fma(p, z, c[i]);
+  return p;
+}
+// sin(h + l), cos(h + l), |h| <= pi/4 (+ rounding), |l| <= ulp(h)
+static inline double sin_kernel(double h, double l) {
+  const double z = h * h;
+  const double s = (h * z) * trig_poly(sin_coef, z);
+  return h + std::fma(l, std::fma(-0.5, z, 1.0), s);
+}
+static inline double cos_kernel(double h, double l) {
+  const double z = h * h;
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  const double e = (1.0 - w) - hz;
+  return w + (std::fma(z * z, trig_poly(cos_coef, z), e) - h * l);
+}
+// round-to-nearest sin and cos of x in [0, 8]
+static inline void sincos_rn(double x, double* sn, double* cs) {
+  const double k = std::nearbyint(x * two_over_pi);
+  const double r1 = std::fma(-k, trig_p1, x);  // exact for 0 <= x < 8
+  const double th = k * trig_p2;
+  const double tl = std::fma(k, trig_p2, -th);
+  const double sum = r1 - th;                  // two-sum of r1 and -th
+  const double bb = sum - r1;
+  const double err = (r1 - (sum - bb)) + (-th - bb);
+  const double lo = err - std::fma(k, trig_p3, tl);
+  const double h = sum + lo;
+  const double l = lo - (h - sum);
+  const double sk = sin_kernel(h, l), ck = cos_kernel(h, l);
+  const int q = (int)k & 3;
+  *sn = q == 0 ? sk : q == 1 ? ck : q == 2 ? -sk : -ck;
+  *cs = q == 0 ? ck : q == 1 ? -sk : q == 2 ? -ck : sk;
+}
+static inline double sin_rn(double x) { double s, c; sincos_rn(x, &s, &c); return s; }
+static inline double cos_rn(double x) { double s, c; sincos_rn(x, &s, &c); return c; }
+static inline double acos_rn(double x) {
+  const double ax = std::fabs(x);
+  if (ax <= 0.5) {
+    const double z = x * x;
+    const double q = (x * z) * asin_poly_eval(z);
+    return asin_pio2_hi - (x - (asin_pio2_lo - q));
+  }
+  const double t = 0.5 - 0.5 * ax;
+  const double s = std::sqrt(t);
+  const double c = t > 0.0 ? std::fma(-s, s, t) / (s + s) : 0.0;
+  const double w = std::fma(s * t, asin_poly_eval(t), c);
+  if (x > 0.0) return 2.0 * s + 2.0 * w;
+  return 2.0 * asin_pio2_hi - ((2.0 * s + 2.0 * w) - 2.0 * asin_pio2_lo);
+}
+static inline double steps_toward(double y, double toward, int n) {
+  for (int i = 0; i < n; ++i) y = std::nextafter(y, toward);
+  return y;
+}
+// interval.cuh:376-418
+static inline double dsin_rd(double x) { double y = steps_toward(sin_rn(x), -2.0, SC_STEPS); return y < -1.0 ? -1.0 : y; }
+static inline double dsin_ru(double x) { double y = steps_toward(sin_rn(x), 2.0, SC_STEPS); return y > 1.0 ? 1.0 : y; }
+static inline double dcos_rd(double x) { double y = steps_toward(cos_rn(x), -2.0, SC_STEPS); return y < -1.0 ? -1.0 : y; }
+static inline double dcos_ru(double x) { double y = steps_toward(cos_rn(x), 2.0, SC_STEPS); return y > 1.0 ? 1.0 : y; }
+static inline double dacos_ru(double x) { double y = steps_toward(acos_rn(x), 4.0, SC_STEPS); return y > ub_pi ? ub_pi : y; }
+static inline double dacos_rd(double x) { double y = steps_toward(acos_rn(x), -1.0, SC_STEPS); return y < 0.0 ? 0.0 : y; }
+
+// interval.cuh:440-478
+static inline IV sin_in_range(IV x) {
+  if (x.ub <= ub_pi_two) {
+    if (x.ub >= lb_pi_two) {
+      double lb1 = dsin_rd(x.lb), lb2 = dsin_rd(ub_pi_two);
+      return IV(lb1 < lb2 ? lb1 : lb2, 1.0);
+    }
+    return IV(dsin_rd(x.lb), dsin_ru(x.ub));
+  }
+  if (x.ub <= ub_3pi_two) {
+    if (x.lb >= ub_pi_two) return IV(x.ub >= lb_3pi_two ? -1.0 : dsin_rd(x.ub), dsin_ru(x.lb));
+    double lb1 = dsin_rd(x.lb), lb2 = dsin_rd(x.ub);
+    return IV(lb1 < lb2 ? lb1 : lb2, 1.0);
+  }
+  if (x.lb <= ub_pi_two) return IV(-1.0, 1.0);
+  if (x.lb <= ub_3pi_two) {
+    double ub1 = dsin_ru(x.lb), ub2 = dsin_ru(x.ub);
+    return IV(-1.0, ub1 > ub2 ? ub1 : ub2);
+  }
+  return IV(dsin_rd(x.lb), dsin_ru(x.ub));
+}
+// interval.cuh:480-499
+static inline IV cos_in_range(IV x) {
+  if (x.ub <= ub_pi) return IV(x.ub >= lb_pi ? -1.0 : dcos_rd(x.ub), dcos_ru(x.lb));
+  if (x.lb <= ub_pi) {
+    double ub1 = dcos_ru(x.lb), ub2 = dcos_ru(x.ub);
+    return IV(-1.0, ub1 > ub2 ? ub1 : ub2);
+  }
+  return IV(dcos_rd(x.lb), dcos_ru(x.ub));
+}
+// interval.cuh:501-576
+template <bool COS>
+static inline IV trig_non_negative(IV x) {
+  if (x.lb >= 0.0 && x.ub <= ub_two_pi) return COS ? cos_in_range(x) : sin_in_range(x);
+  const double li = std::trunc(div_rd(x.lb, ub_two_pi));
+  const double ui = std::trunc(div_ru(x.ub, lb_two_pi));
+  const double diff = ui - li;
+  if (diff > 1.0) return IV(-1.0, 1.0);
+  if (diff == 0.0) {
+    double lr = add_rd(x.lb, -mul_ru(li, ub_two_pi));
+    double ur = add_ru(x.ub, -mul_rd(li, lb_two_pi));
+    lr = lr < 0.0 ? 0.0 : (lr > ub_two_pi ? ub_two_pi : lr);
+    ur = ur < 0.0 ? 0.0 : (ur > ub_two_pi ? ub_two_pi : ur);
+    return COS ? cos_in_range(IV(lr, ur)) : sin_in_range(IV(lr, ur));
+  }
+  double lr = add_rd(x.lb, -mul_ru(li, ub_two_pi));
+  if (lr < 0.0) return IV(-1.0, 1.0);
+  if (lr > ub_two_pi) lr = ub_two_pi;
+  double ur = add_ru(x.ub, -mul_rd(ui, lb_two_pi));
+  if (ur > ub_two_pi) return IV(-1.0, 1.0);
+  if (ur < 0.0) ur = 0.0;
+  IV a = COS ? cos_in_range(IV(lr, ub_two_pi)) : sin_in_range(IV(lr, ub_two_pi));
+  IV b = COS ? cos_in_range(IV(0.0, ur)) : sin_in_range(IV(0.0, ur));
+  return a.do_join(b);
+}
+// interval.cuh:578-610
+static inline IV sin(const IV& x) {
+  if (x.lb >= 0.0) return trig_non_negative<false>(x);
+  if (x.ub <= 0.0) return -trig_non_negative<false>(-x);
+  IV p = trig_non_negative<false>(IV(0.0, x.ub));
+  return p.do_join(-trig_non_negative<false>(IV(0.0, -x.lb)));
+}
+static inline IV cos(const IV& x) {
+  if (x.lb >= 0.0) return trig_non_negative<true>(x);
+  if (x.ub <= 0.0) return trig_non_negative<true>(-x);
+  const double nl = -x.lb;
+  return trig_non_negative<true>(IV(0.0, nl < x.ub ? x.ub : nl));
+}
+static inline IV acos(const IV& x) {
+  if (x.lb < -1.0 || x.ub > 1.0) throw Trap{TRAP_ASIN_DOMAIN};
+  return IV(dacos_rd(x.ub), dacos_ru(x.lb));
+}
+// interval.cuh:227-235; for a negative interval the reference's
+// {rcp_rd(lb), rcp_ru(ub)} is inverted, the enclosure [rd(1/ub), ru(1/lb)] is used.
+static inline IV reciprocal(const IV& x) {
+  if (x.lb > 0.0 || x.ub < 0.0) return IV(div_rd(1.0, x.ub), div_ru(1.0, x.lb));
+  const double inf = DBL_MAX * 2.0;
+  return IV(-inf, inf);
+}
+
+// ---------------------------------------------------------------------------
 // get_subinterval (subintervals.hpp:31-42).  nvcc contracts lb + diff*i into a
 // fused multiply-add by default; the restatement writes the fma explicitly.
 // ---------------------------------------------------------------------------

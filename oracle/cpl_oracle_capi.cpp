@@ -221,9 +221,56 @@ int cplo_prim(int op, const double* a, const double* b, double* out, size_t n) {
       case 23: r = div_ru32((float)x, (float)y); break;
       case 27: r = f32_rd(x); break;
       case 28: r = f32_ru(x); break;
+      case 29: r = sin_rn(x); break;
+      case 30: r = cos_rn(x); break;
+      case 31: r = acos_rn(x); break;
+      case 32: r = dsin_rd(x); break;
+      case 33: r = dsin_ru(x); break;
+      case 34: r = dcos_rd(x); break;
+      case 35: r = dcos_ru(x); break;
+      case 36: r = dacos_rd(x); break;
+      case 37: r = dacos_ru(x); break;
       default: return CPL_ERR_ARG;
     }
     out[i] = r;
+  }
+  return CPL_OK;
+}
+
+// Elementwise interval operations (same op codes as cpl_interval_op).
+int cplo_interval_op(int op, const double* a, const double* b, double* out, int* trap, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    const IV x(a[2 * i], a[2 * i + 1]);
+    const IV y = b ? IV(b[2 * i], b[2 * i + 1]) : IV(0.0, 0.0);
+    IV r(0.0, 0.0);
+    int t = 0;
+    try {
+      switch (op) {
+        case 0: r = x + y; break;
+        case 1: r = x - y; break;
+        case 2: r = x * y; break;
+        case 3: r = x / y; break;
+        case 4: r = sqrt(x); break;
+        case 5: r = x.square(); break;
+        case 6: r = x.abs(); break;
+        case 7: r = reciprocal(x); break;
+        case 8: r = x / y.lb; break;
+        case 9: r = IV(x.lb, x.lb) / y; break;
+        case 10: r = sin(x); break;
+        case 11: r = cos(x); break;
+        case 12: r = asin(x); break;
+        case 13: r = acos(x); break;
+        case 14: r = x * y.lb; break;
+        case 15: r = -x; break;
+        default: return CPL_ERR_ARG;
+      }
+    } catch (const Trap& e) {
+      t = e.code;
+      r = IV(0.0, 0.0);
+    }
+    out[2 * i] = r.lb;
+    out[2 * i + 1] = r.ub;
+    trap[i] = t;
   }
   return CPL_OK;
 }

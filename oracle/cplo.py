@@ -33,6 +33,7 @@ def lib(counting=False):
         L = ctypes.CDLL(path)
         L.cplo_prim.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.c_size_t]
         L.cplo_geom.argtypes = [ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
+        L.cplo_interval_op.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
         L.cplo_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
         L.cplo_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8,
                                       ctypes.c_int]
@@ -54,7 +55,7 @@ def prim(op, a, b=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     out = np.empty_like(a)
     bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
-    code = abi.PRIM_OPS[op] if op in abi.PRIM_OPS else abi.PRIM32_OPS[op]
+    code = abi.prim_code(op)
     rc = lib().cplo_prim(code, _dptr(a), None if bb is None else _dptr(bb), _dptr(out), a.size)
     assert rc == 0
     return out
@@ -121,3 +122,16 @@ def first_problematic(family, outer_la, subdiv, nthreads=0):
     rc = lib().cplo_first_problematic(family, outer_la[0], outer_la[1], sd, ctypes.byref(first), ctypes.byref(ev),
                                       ctypes.byref(trap), nthreads)
     return rc, first.value, ev.value, trap.value
+
+
+def interval_op(op, a, b=None):
+    """Oracle counterpart of circlepacking_amd.interval_op."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 2)
+    n = a.shape[0]
+    bb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), dtype=np.float64)
+    out = np.zeros((n, 2), dtype=np.float64)
+    trap = np.zeros(n, dtype=np.int32)
+    rc = lib().cplo_interval_op(abi.IV_OPS[op], _dptr(a), None if bb is None else _dptr(bb), _dptr(out),
+                                trap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+    assert rc == 0
+    return out, trap
