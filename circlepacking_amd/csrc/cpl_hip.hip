@@ -187,11 +187,31 @@ __global__ void k_la_table(double box_lb, double box_ub, int n0, int levels, int
 
 __device__ __forceinline__ uint64_t la_level_offset(int n0, int d) { return (uint64_t)n0 * ((1ull << d) - 1); }
 
-// wave-wide sum of a 64-bit counter, lane 0 adds it to *dst
-__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+// Statistics copy of (round d, family F) for slot `slot`.
+__device__ __forceinline__ RoundStats* stat_slot(const RefineParams& P, int d, int F, uint32_t slot) {
+  return P.stats + ((size_t)(d * CPL_NUM_FAMILIES + F) * STAT_SLOTS + (slot & (STAT_SLOTS - 1)));
+}
+__device__ __forceinline__ uint32_t wave_slot() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+// Block-wide sum of per-thread counters c[16] (RoundStats word order) added to
+// *dst with one atomic per nonzero word: waves reduce by shuffles, wave
+// partials meet in LDS.  Called once, at the end of a kernel, by every thread.
+__device__ __forceinline__ void block_stats_add(RoundStats* dst, const unsigned long long* c) {
+  __shared__ unsigned long long part[16][16];  // [wave][word], blocks of up to 1024 threads
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+  for (int k = 0; k < 16; ++k) {
+    unsigned long long v = c[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) part[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    unsigned long long sum = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) sum += part[i][threadIdx.x];
+    if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(dst) + threadIdx.x, sum);
+  }
 }
 
 __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, uint32_t bits) {
@@ -321,7 +341,8 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
       c_f32t += tried ? 1 : 0;
       int v = vb & 7;
       c_eval += 1;
-      c_v[v] += 1;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c_v[k] += (v == k) ? 1 : 0;
       if (P.digest) {
         uint64_t h = item_hash(e, root, F, meta);
         c_dig += h;
@@ -408,18 +429,9 @@ __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint6
       }
     }
   }
-  RoundStats* S = P.stats + P.depth * CPL_NUM_FAMILIES + F;
-  wave_add(&S->evaluations, c_eval);
-#pragma unroll
-  for (int v = 0; v < 8; ++v) wave_add(&S->verdicts[v], c_v[v]);
-  wave_add(&S->digest, c_dig);
-  wave_add(&S->vdigest, c_vdig);
-  wave_add(&S->certified, c_cert);
-  wave_add(&S->unresolved, c_unres);
-  if (EXT) {
-    wave_add(&S->fp32_settled, c_f32);
-    wave_add(&S->fp32_tried, c_f32t);
-  }
+  const unsigned long long acc[16] = {c_eval, c_v[0], c_v[1], c_v[2], c_v[3], c_v[4], c_v[5], c_v[6],
+                                      c_v[7], c_dig, c_vdig, c_cert, c_unres, c_f32, c_f32t, 0};
+  block_stats_add(stat_slot(P, P.depth, F, blockIdx.x), acc);
 }
 
 // Root round with admissibility decided once per (la, r1, r2) triple: an
@@ -507,12 +519,21 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
       }
     }
   }
-  RoundStats* S = P.stats + 0 * CPL_NUM_FAMILIES + F;
-  wave_add(&S->evaluations, c_ood);
-  wave_add(&S->verdicts[V_OOD], c_ood);
-  wave_add(&S->certified, c_ood);
-  wave_add(&S->digest, c_dig);
-  wave_add(&S->vdigest, c_vdig);
+  unsigned long long acc[16] = {c_ood, 0, 0, 0, 0, 0, 0, 0, 0, c_dig, c_vdig, c_ood, 0, 0, 0, 0};
+  acc[1 + V_OOD] = c_ood;
+  block_stats_add(stat_slot(P, 0, F, blockIdx.x), acc);
+}
+
+// Sum the STAT_SLOTS copies of every (round, family) record into out.
+__global__ void k_reduce_stats(const RoundStats* __restrict__ in, RoundStats* __restrict__ out) {
+  constexpr int W = sizeof(RoundStats) / sizeof(unsigned long long);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (record, word)
+  if (i >= CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * W) return;
+  const int rec = i / W, w = i % W;
+  const unsigned long long* src = reinterpret_cast<const unsigned long long*>(in + (size_t)rec * STAT_SLOTS) + w;
+  unsigned long long acc = 0;
+  for (int k = 0; k < STAT_SLOTS; ++k) acc += src[(size_t)k * W];
+  reinterpret_cast<unsigned long long*>(out + rec)[w] = acc;
 }
 
 // Fill n words with a value (graph-safe replacement of hipMemsetAsync).
@@ -735,41 +756,66 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
     count = base;
     bool split = false;
     const uint8_t pb = kNoProposal;  // the learned proposal runs with the round-based forms only
+    int depth = 0, v = 0, cert = 0;
+    bool hit = false, tried = false;
+    uint64_t h = 0, hv = 0;
     if (valid) {
-      const int depth = (int)(meta >> 16);
+      depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      bool hit = false, tried = false;
       const bool pre = EXT && ((P.prefilter_depths >> depth) & 1u) && (pb & 1);
       uint8_t vb = eval_synthetic<F, EXT>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0,
                                           lbrsp, pre, &hit, &tried);
-      const int v = vb & 7;
-      unsigned long long* st = sst + depth * ST_N;
-      atomicAdd(st + ST_EVAL, 1ull);
-      if (hit) atomicAdd(st + ST_F32, 1ull);
-      if (tried) atomicAdd(st + ST_F32T, 1ull);
-      atomicAdd(st + ST_V + v, 1ull);
+      v = vb & 7;
       if (P.digest) {
-        uint64_t h = item_hash(e, root, F, meta);
-        atomicAdd(st + ST_DIG, (unsigned long long)h);
-        atomicAdd(st + ST_VDIG, (unsigned long long)mix64(h ^ vb));
+        h = item_hash(e, root, F, meta);
+        hv = mix64(h ^ vb);
       }
       uint32_t fl = (vb & 8) ? CPL_RF_NOFIT_FLAG : 0;
       if (v == V_CRITICAL && depth < P.max_depth) {
         split = true;
       } else {
-        int cert = 0;
         switch (v) {
           case V_PROVEN: fl |= CPL_RF_PROVEN; cert = 1; break;
           case V_EMPTY: fl |= CPL_RF_EMPTY; cert = 1; break;
           case V_OOD: fl |= CPL_RF_OOD; cert = 1; break;
-          case V_CRITICAL: fl |= CPL_RF_CRITICAL; break;
-          case V_NOFIT: fl |= CPL_RF_NOFIT; break;
-          default: fl |= CPL_RF_TRAP; break;
+          case V_CRITICAL: fl |= CPL_RF_CRITICAL; cert = 2; break;
+          case V_NOFIT: fl |= CPL_RF_NOFIT; cert = 2; break;
+          default: fl |= CPL_RF_TRAP; cert = 2; break;
         }
-        atomicAdd(st + (cert ? ST_CERT : ST_UNRES), 1ull);
       }
       if (fl && P.root_flags) flag_root(P.root_flags, (uint64_t)root * P.nfam + P.fam_slot, fl);
+    }
+    // ---- statistics: per depth present in the batch, counts from ballots, sums
+    // by shuffles; the block is one wave, so its LDS record needs no atomics ----
+    for (uint64_t rem = __ballot(valid); rem;) {
+      const int d0 = __shfl(depth, __ffsll((unsigned long long)rem) - 1, 64);
+      const bool in = valid && depth == d0;
+      const uint64_t m = __ballot(in);
+      rem &= ~m;
+      unsigned long long cnt[ST_N];
+      cnt[ST_EVAL] = __popcll(m);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cnt[ST_V + k] = __popcll(__ballot(in && v == k));
+      cnt[ST_CERT] = __popcll(__ballot(in && cert == 1));
+      cnt[ST_UNRES] = __popcll(__ballot(in && cert == 2));
+      cnt[ST_F32] = __popcll(__ballot(in && hit));
+      cnt[ST_F32T] = __popcll(__ballot(in && tried));
+      unsigned long long dg = in ? h : 0, dv = in ? hv : 0;
+      if (P.digest) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          dg += __shfl_xor(dg, off, 64);
+          dv += __shfl_xor(dv, off, 64);
+        }
+      }
+      cnt[ST_DIG] = dg;
+      cnt[ST_VDIG] = dv;
+      if (lane == 0) {
+        unsigned long long* st = sst + d0 * ST_N;
+#pragma unroll
+        for (int k = 0; k < ST_N; ++k) st[k] += cnt[k];
+      }
     }
     // ---- push children ----
     const uint64_t mask = __ballot(split);
@@ -806,11 +852,12 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
     }
   }
   // flush the wave's statistics
+  __syncthreads();
   for (int k = lane; k < CPL_MAX_ROUNDS * ST_N; k += 64) {
     unsigned long long v = sst[k];
     if (!v) continue;
     const int d = k / ST_N, c = k % ST_N;
-    RoundStats* S = P.stats + d * CPL_NUM_FAMILIES + F;
+    RoundStats* S = stat_slot(P, d, F, blockIdx.x);
     unsigned long long* dst = c == ST_EVAL ? &S->evaluations
                             : c < ST_DIG   ? &S->verdicts[c - ST_V]
                             : c == ST_DIG  ? &S->digest
@@ -932,7 +979,7 @@ __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
   }
   if ((threadIdx.x & 63) == 0) {
     if (best != ~0ull) atomicMin(P.first, best);
-    if (ev) atomicAdd(P.evaluated, ev);
+    if (ev) atomicAdd(P.evaluated + 16 * (((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & 63), ev);
   }
 }
 
@@ -954,7 +1001,7 @@ struct Context {
   hipEvent_t fork_ev = nullptr, join_ev[CPL_NUM_FAMILIES] = {};
   DevBuf scratch_in, scratch_in2, scratch_out, cells, verdicts;
   DevBuf wl[2];           // ping-pong worklists; each holds nfam regions
-  DevBuf la_table, root_flags, stats, counters;
+  DevBuf la_table, root_flags, stats, stats_sum, counters;
   DevBuf mr_ranges, mr_misc, mr_crit;
   DevBuf persist_q;
   DevBuf mlp;                 // MlpPacked per family
@@ -1095,7 +1142,7 @@ int cpl_shutdown(void) {
   if (!g.ready) return CPL_OK;
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf* b : {&g.scratch_in, &g.scratch_in2, &g.scratch_out, &g.cells, &g.verdicts, &g.wl[0], &g.wl[1],
-                    &g.la_table, &g.root_flags, &g.stats, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit,
+                    &g.la_table, &g.root_flags, &g.stats, &g.stats_sum, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit,
                     &g.persist_q, &g.mlp, &g.records, &g.prop})
     b->release();
   (void)hipStreamDestroy(g.stream);
@@ -1279,7 +1326,7 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.block = cfg->block > 0 ? cfg->block : 256;
   if (pl.block % 64 || pl.block > 1024) return fail(CPL_ERR_ARG, "block must be a multiple of 64 up to 1024");
   pl.cpt = cfg->cells_per_thread > 0 ? cfg->cells_per_thread : 1;
-  pl.stats_bytes = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats);
+  pl.stats_bytes = (size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * STAT_SLOTS * sizeof(RoundStats);
   pl.flag_words = (pl.n_roots * pl.nf + 3) / 4;
   return CPL_OK;
 }
@@ -1292,6 +1339,7 @@ int ensure_buffers(const Plan& pl) {
   if ((rc = g.la_table.ensure(pl.la_entries * 3 * sizeof(double)))) return rc;
   if ((rc = g.root_flags.ensure(pl.flag_words * 4 + 4))) return rc;
   if ((rc = g.stats.ensure(pl.stats_bytes))) return rc;
+  if ((rc = g.stats_sum.ensure(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats)))) return rc;
   if ((rc = g.counters.ensure(4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
   for (int b = 0; b < 2; ++b)
     if ((rc = g.wl[b].ensure((size_t)pl.cap * per_item * pl.nf))) return rc;
@@ -1359,7 +1407,14 @@ template <int F> struct LaunchTriples {
 
 int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   std::vector<RoundStats> rs(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES);
-  HIP_TRY(hipMemcpy(rs.data(), g.stats.p, pl.stats_bytes, hipMemcpyDeviceToHost));
+  {
+    const int words = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * (int)(sizeof(RoundStats) / 8);
+    hipLaunchKernelGGL(k_reduce_stats, dim3((words + 255) / 256), dim3(256), 0, g.stream, g.stats.as<RoundStats>(),
+                       g.stats_sum.as<RoundStats>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(rs.data(), g.stats_sum.p, rs.size() * sizeof(RoundStats), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+  }
   if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, pl.n_roots * pl.nf, hipMemcpyDeviceToHost));
   double seconds = stats->seconds, eval_seconds = stats->eval_seconds;
   uint64_t peak = stats->peak_items;
@@ -1869,7 +1924,8 @@ int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, in
   if (!(family == 0 || family == 2 || family == 3 || family == 5)) n[3] = 1;
   for (int k = 0; k < 4; ++k)
     if (n[k] < 1) return fail(CPL_ERR_ARG, "subdivision counts must be >= 1");
-  if ((rc = g.mr_ranges.ensure((size_t)n[0] * 4 * sizeof(double))) || (rc = g.mr_misc.ensure(256))) return rc;
+  if ((rc = g.mr_ranges.ensure((size_t)n[0] * 4 * sizeof(double))) || (rc = g.mr_misc.ensure(256 + 64 * 128)))
+    return rc;
   const uint32_t crit_cap = output_criticals ? (1u << 20) : 0u;
   if (crit_cap && (rc = g.mr_crit.ensure((size_t)crit_cap * 9 * sizeof(double)))) return rc;
 
@@ -1879,7 +1935,7 @@ int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, in
   P.r1_ranges = g.mr_ranges.as<double>();
   char* misc = g.mr_misc.as<char>();
   P.first = reinterpret_cast<unsigned long long*>(misc);
-  P.evaluated = reinterpret_cast<unsigned long long*>(misc + 8);
+  P.evaluated = reinterpret_cast<unsigned long long*>(misc + 256);  // 64 slots, one 128-byte line each
   P.status = reinterpret_cast<uint32_t*>(misc + 16);
   P.crit_count = reinterpret_cast<uint32_t*>(misc + 24);
   P.output_criticals = output_criticals;
@@ -1898,6 +1954,7 @@ int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, in
     init[3] = 0;
   }
   HIP_TRY(hipMemcpyAsync(misc, init, sizeof(init), hipMemcpyHostToDevice, g.stream));
+  HIP_TRY(hipMemsetAsync(misc + 256, 0, 64 * 128, g.stream));
   hipLaunchKernelGGL(k_mode_r_ranges, dim3((unsigned)((n[0] + 255) / 256)), dim3(256), 0, g.stream, P.outer, n[0],
                      g.mr_ranges.as<double>());
   HIP_TRY(hipGetLastError());
@@ -1905,8 +1962,12 @@ int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, in
   dispatch_family<LaunchModeR>(family, grid_for(total, 256), P);
   HIP_TRY(hipGetLastError());
   unsigned long long back[4];
+  std::vector<unsigned long long> evs(64 * 16);
   HIP_TRY(hipMemcpyAsync(back, misc, sizeof(back), hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipMemcpyAsync(evs.data(), misc + 256, evs.size() * 8, hipMemcpyDeviceToHost, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
+  back[1] = 0;
+  for (int k = 0; k < 64; ++k) back[1] += evs[16 * k];
   {
     uint64_t v = back[0];
     uint64_t mask = ((v >> 63) - 1) | (1ull << 63);

@@ -20,6 +20,12 @@ struct Worklist {
   uint32_t* meta;
 };
 
+// Per (round, family) statistics are kept in STAT_SLOTS copies (one 128-byte
+// line each) and each wave adds into the copy of its index: tens of thousands
+// of waves adding to one line serialise in L2 and cost as much as the
+// evaluation itself.  The copies are summed on the device after the run.
+constexpr int STAT_SLOTS = 64;
+
 struct RoundStats {
   unsigned long long evaluations;
   unsigned long long verdicts[8];
@@ -31,6 +37,7 @@ struct RoundStats {
   unsigned long long fp32_tried;    // items the FP32 prefilter ran on
   unsigned long long pad;
 };
+static_assert(sizeof(RoundStats) == 16 * sizeof(unsigned long long), "16 words, RoundStats word order used by kernels");
 
 struct RefineParams {
   int32_t grid[4];
