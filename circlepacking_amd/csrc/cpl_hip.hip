@@ -290,6 +290,7 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
 template <int F, bool ROOT, int LB = 256, bool EXT = false>
 __global__ __launch_bounds__(LB) void k_round(RefineParams P, Worklist in, uint64_t n) {
   if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
+  if (P.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *P.zero_count = 0u;
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
@@ -1331,6 +1332,10 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   return CPL_OK;
 }
 
+// Counter words: sets 0, 1, 4 round counters, set 2 live root items, then the
+// overflow word.
+constexpr size_t kCounterBytes = 6 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64;
+
 // Device buffers: worklists [0], [1] (ping-pong) and [2] (live root items),
 // each with one region per family; counters[3][families] + overflow word.
 int ensure_buffers(const Plan& pl) {
@@ -1340,7 +1345,7 @@ int ensure_buffers(const Plan& pl) {
   if ((rc = g.root_flags.ensure(pl.flag_words * 4 + 4))) return rc;
   if ((rc = g.stats.ensure(pl.stats_bytes))) return rc;
   if ((rc = g.stats_sum.ensure(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats)))) return rc;
-  if ((rc = g.counters.ensure(4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64))) return rc;
+  if ((rc = g.counters.ensure(kCounterBytes))) return rc;
   for (int b = 0; b < 2; ++b)
     if ((rc = g.wl[b].ensure((size_t)pl.cap * per_item * pl.nf))) return rc;
   if (pl.learned && (rc = g.prop.ensure((size_t)pl.cap * pl.nf))) return rc;
@@ -1349,7 +1354,14 @@ int ensure_buffers(const Plan& pl) {
 }
 
 uint32_t* counter(int buf, int slot) { return g.counters.as<uint32_t>() + buf * CPL_NUM_FAMILIES + slot; }
-uint32_t* overflow_word() { return g.counters.as<uint32_t>() + 3 * CPL_NUM_FAMILIES; }
+uint32_t* overflow_word() { return g.counters.as<uint32_t>() + 5 * CPL_NUM_FAMILIES; }
+// Round counters of the graph path: round d reads rot_counter(d) and writes
+// rot_counter(d + 1), and clears rot_counter(d + 2) for the next round (it was
+// the input count of round d - 1), so no clearing node runs between rounds.
+uint32_t* rot_counter(int d, int slot) {
+  static const int set[3] = {0, 1, 4};
+  return counter(set[d % 3], slot);
+}
 
 RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   RefineParams P;
@@ -1457,7 +1469,7 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   std::vector<int> kslot;
   if (pl.flags) HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, pl.flag_words * 4, g.stream));
   HIP_TRY(hipMemsetAsync(g.stats.p, 0, pl.stats_bytes, g.stream));
-  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
+  HIP_TRY(hipMemsetAsync(g.counters.p, 0, kCounterBytes, g.stream));
   HIP_TRY(hipEventRecord(ev0, g.stream));
   hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
                      cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
@@ -1535,32 +1547,45 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
 // Fork: every family branch waits for the work enqueued so far on the library
 // stream; join: the library stream waits for every branch.  Inside a stream
 // capture this yields a graph with one independent chain per family.
+// Stream of family slot s: slot 0 runs on the library stream itself, so four
+// families use four streams (the hardware offers four queues per process by
+// default; a fifth stream would share a queue and serialise two families).
+hipStream_t fam_stream_of(int s) { return s == 0 ? g.stream : g.fam_stream[s - 1]; }
+
 void fork_families(int nf) {
   (void)hipEventRecord(g.fork_ev, g.stream);
-  for (int s = 0; s < nf; ++s) (void)hipStreamWaitEvent(g.fam_stream[s], g.fork_ev, 0);
+  for (int s = 1; s < nf; ++s) (void)hipStreamWaitEvent(fam_stream_of(s), g.fork_ev, 0);
 }
 void join_families(int nf) {
-  for (int s = 0; s < nf; ++s) {
-    (void)hipEventRecord(g.join_ev[s], g.fam_stream[s]);
+  for (int s = 1; s < nf; ++s) {
+    (void)hipEventRecord(g.join_ev[s], fam_stream_of(s));
     (void)hipStreamWaitEvent(g.stream, g.join_ev[s], 0);
   }
   g_launch_stream = nullptr;
 }
 
 // ---- graph path ---------------------------------------------------------------
+// Captured graphs: the prologue (library stream) and one linear chain per
+// family slot (launched on that family's stream, so the chains run
+// concurrently; a single graph with parallel branches started them one after
+// another).
 struct GraphCache {
   bool valid = false;
   cpl_refine_cfg cfg;
   bool flags = false;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec[1 + CPL_NUM_FAMILIES] = {};
+  hipGraph_t graph[1 + CPL_NUM_FAMILIES] = {};
+  int n = 0;
   // expected items per (round, slot) from the previous replay: sizes the grids
   std::vector<uint32_t> expect;
   void reset() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    for (int k = 0; k < 1 + CPL_NUM_FAMILIES; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      exec[k] = nullptr;
+      graph[k] = nullptr;
+    }
+    n = 0;
     valid = false;
   }
 };
@@ -1573,71 +1598,82 @@ unsigned grid_tuned(uint64_t expect, const Plan& pl) {
   return (unsigned)std::max<uint64_t>(1, std::min(want, cap));
 }
 
-// Enqueue the whole run (all families, all rounds) with device-side sizes.
-void enqueue_all(const cpl_refine_cfg* cfg, const Plan& pl, const std::vector<uint32_t>& expect) {
-  const bool hier = (cfg->options & CPL_OPT_HIER_ROOTS) != 0;
+// Work shared by every family (cleared statistics, counters and flags, the la
+// table), enqueued on the library stream.
+void enqueue_prologue(const cpl_refine_cfg* cfg, const Plan& pl) {
+  g_launch_stream = nullptr;
   fill(g.stats.p, pl.stats_bytes / 4, 0u);
-  fill(g.counters.p, 3 * CPL_NUM_FAMILIES + 1, 0u);
+  fill(g.counters.p, 5 * CPL_NUM_FAMILIES + 1, 0u);
   if (pl.flags) fill(g.root_flags.p, pl.flag_words, 0u);
   hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
                      cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
+}
+
+// Round d of family slot s with device-side sizes, enqueued on that family's
+// stream (a family's rounds form one linear chain; families are independent).
+void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, const std::vector<uint32_t>& expect) {
+  const bool hier = (cfg->options & CPL_OPT_HIER_ROOTS) != 0;
   RefineParams P = base_params(cfg, pl);
-  auto exp_of = [&](int d, int s) -> uint64_t {
-    size_t k = (size_t)d * CPL_NUM_FAMILIES + s;
+  auto exp_of = [&](int dd) -> uint64_t {
+    size_t k = (size_t)dd * CPL_NUM_FAMILIES + s;
     return k < expect.size() && expect[k] ? expect[k] : pl.cap;
   };
-  fork_families(pl.nf);
-  for (int s = 0; s < pl.nf; ++s) {
-    g_launch_stream = g.fam_stream[s];
-    set_family(P, pl, s);
-    P.depth = 0;
+  g_launch_stream = fam_stream_of(s);
+  set_family(P, pl, s);
+  P.depth = d;
+  if (d == 0) {
     P.in_count = nullptr;
-    Worklist none{};
-    // round 0 -> buffer 1
+    P.out = region(g.wl[1], pl.cap, s);
+    P.out_count = rot_counter(1, s);
+    P.zero_count = rot_counter(2, s);
+    P.prop = nullptr;  // the root round runs without proposals in every form
     if (hier) {
       // live root items go to buffer 0, evaluated from there into buffer 1
       RefineParams T = P;
       T.out = region(g.wl[0], pl.cap, s);
       T.out_count = counter(2, s);
+      T.zero_count = nullptr;
       dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
       P.in_count = counter(2, s);
-      P.out = region(g.wl[1], pl.cap, s);
-      P.out_count = counter(1, s);
-      P.prop = nullptr;  // the root round runs without proposals in every form
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(0, s), pl), pl.block, P,
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(0), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0);
     } else {
-      P.out = region(g.wl[1], pl.cap, s);
-      P.out_count = counter(1, s);
-      P.prop = nullptr;
-      dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.ext, grid_tuned(pl.n_roots, pl), pl.block, P, none, pl.n_roots);
+      Worklist none{};
+      dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.ext, grid_tuned(pl.n_roots, pl), pl.block, P, none,
+                                    pl.n_roots);
     }
-    int cur = 1;
-    for (int d = 1; d <= cfg->max_depth; ++d) {
-      const int nxt = cur ^ 1;
-      P.depth = d;
-      fill(counter(nxt, s), 1, 0u);
-      P.in_count = counter(cur, s);
-      P.out = region(g.wl[nxt], pl.cap, s);
-      P.out_count = counter(nxt, s);
-      launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, grid_tuned(exp_of(d, s), pl));
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d, s), pl), pl.block, P,
-                                    region(g.wl[cur], pl.cap, s), (uint64_t)0);
-      cur = nxt;
-    }
+  } else {
+    // round d reads buffer 1 - d % 2 ... rounds alternate buffers 1 -> 0 -> 1
+    const int cur = d % 2 == 1 ? 1 : 0, nxt = cur ^ 1;
+    P.in_count = rot_counter(d, s);
+    P.out = region(g.wl[nxt], pl.cap, s);
+    P.out_count = rot_counter(d + 1, s);
+    P.zero_count = rot_counter(d + 2, s);
+    launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, grid_tuned(exp_of(d), pl));
+    dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d), pl), pl.block, P,
+                                  region(g.wl[cur], pl.cap, s), (uint64_t)0);
   }
-  join_families(pl.nf);
+  g_launch_stream = nullptr;
+}
+
+void enqueue_family(const cpl_refine_cfg* cfg, const Plan& pl, int s, const std::vector<uint32_t>& expect) {
+  for (int d = 0; d <= cfg->max_depth; ++d) enqueue_round(cfg, pl, s, d, expect);
 }
 
 int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   bool same = g_graph.valid && g_graph.flags == pl.flags && std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) == 0;
   if (!same) {
     g_graph.reset();
-    HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_all(cfg, pl, g_graph.expect);
-    hipError_t e = hipStreamEndCapture(g.stream, &g_graph.graph);
-    if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    HIP_TRY(hipGraphInstantiate(&g_graph.exec, g_graph.graph, nullptr, nullptr, 0));
+    for (int k = 0; k <= pl.nf; ++k) {
+      hipStream_t cs = k == 0 ? g.stream : fam_stream_of(k - 1);
+      HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      if (k == 0) enqueue_prologue(cfg, pl);
+      else enqueue_family(cfg, pl, k - 1, g_graph.expect);
+      hipError_t e = hipStreamEndCapture(cs, &g_graph.graph[k]);
+      if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      HIP_TRY(hipGraphInstantiate(&g_graph.exec[k], g_graph.graph[k], nullptr, nullptr, 0));
+    }
+    g_graph.n = pl.nf;
     g_graph.cfg = *cfg;
     g_graph.flags = pl.flags;
     g_graph.valid = true;
@@ -1646,7 +1682,30 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(hipEventRecord(ev0, g.stream));
-  HIP_TRY(hipGraphLaunch(g_graph.exec, g.stream));
+  HIP_TRY(hipGraphLaunch(g_graph.exec[0], g.stream));
+  HIP_TRY(hipEventRecord(g.fork_ev, g.stream));
+  // submit the family chains with the most work (previous replay) first: host
+  // submission of a captured chain takes time, the largest chain bounds the run
+  std::vector<int> order(pl.nf);
+  std::vector<uint64_t> work(pl.nf, 0);
+  for (int s = 0; s < pl.nf; ++s) {
+    order[s] = s;
+    for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
+      const size_t k = (size_t)d * CPL_NUM_FAMILIES + s;
+      if (k < g_graph.expect.size()) work[s] += g_graph.expect[k];
+    }
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+  for (int s : order) {
+    if (s == 0) {
+      HIP_TRY(hipGraphLaunch(g_graph.exec[1], g.stream));
+      continue;
+    }
+    HIP_TRY(hipStreamWaitEvent(fam_stream_of(s), g.fork_ev, 0));
+    HIP_TRY(hipGraphLaunch(g_graph.exec[1 + s], fam_stream_of(s)));
+    HIP_TRY(hipEventRecord(g.join_ev[s], fam_stream_of(s)));
+  }
+  for (int s = 1; s < pl.nf; ++s) HIP_TRY(hipStreamWaitEvent(g.stream, g.join_ev[s], 0));
   HIP_TRY(hipEventRecord(ev1, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
   uint32_t ovf = 0;
@@ -1729,7 +1788,7 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
   HIP_TRY(hipEventCreate(&ev1));
   if (pl.flags) HIP_TRY(hipMemsetAsync(g.root_flags.p, 0, pl.flag_words * 4, g.stream));
   HIP_TRY(hipMemsetAsync(g.stats.p, 0, pl.stats_bytes, g.stream));
-  HIP_TRY(hipMemsetAsync(g.counters.p, 0, 4 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64, g.stream));
+  HIP_TRY(hipMemsetAsync(g.counters.p, 0, kCounterBytes, g.stream));
   HIP_TRY(hipMemsetAsync(g.persist_q.p, 0, (size_t)CPL_NUM_FAMILIES * 8 * sizeof(uint32_t), g.stream));
   HIP_TRY(hipEventRecord(ev0, g.stream));
   hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
@@ -1738,7 +1797,7 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
   P.depth = 0;
   fork_families(pl.nf);
   for (int s = 0; s < pl.nf; ++s) {
-    g_launch_stream = g.fam_stream[s];
+    g_launch_stream = fam_stream_of(s);
     set_family(P, pl, s);
     RefineParams T = P;
     T.out = region(g.wl[0], pl.cap, s);

@@ -65,6 +65,7 @@ struct RefineParams {
   float mlp_threshold;       // gate: the prefilter runs when output 0 exceeds this logit
   int32_t learned_split;     // 1: mode-0 splits take the proposal's best-scored eligible axis
   const uint8_t* prop;       // extended round kernels: proposal byte per input item (k_propose), or null
+  uint32_t* zero_count;      // cleared by the round kernel (the next round's output counter), or null
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
