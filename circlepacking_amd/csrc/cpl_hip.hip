@@ -295,8 +295,11 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
   if (P.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *P.zero_count = 0u;
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  unsigned long long c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
-  unsigned long long c_f32 = 0, c_f32t = 0;
+  // Wave-uniform counters (kept in scalar registers: counts come from ballots,
+  // digests are reduced across the wave each iteration), so no vector
+  // registers are held across the evaluation for statistics.
+  uint64_t c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
+  uint64_t c_f32 = 0, c_f32t = 0;
 
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -305,6 +308,9 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
     uint32_t root = 0, la_idx = 0, meta = 0;
     uint8_t vb = 0;
     bool split = false;
+    bool hit = false, tried = false;
+    int cert = 0;
+    uint64_t h = 0, hv = 0;
     if (valid) {
       if (ROOT) {
         uint64_t r = i;
@@ -336,20 +342,13 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
       int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
-      bool hit = false, tried = false;
       const bool pre = EXT && ((P.prefilter_depths >> depth) & 1u) && (pb & 1);
       vb = eval_synthetic<F, EXT>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp,
                                   pre, &hit, &tried);
-      c_f32 += hit ? 1 : 0;
-      c_f32t += tried ? 1 : 0;
       int v = vb & 7;
-      c_eval += 1;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) c_v[k] += (v == k) ? 1 : 0;
       if (P.digest) {
-        uint64_t h = item_hash(e, root, F, meta);
-        c_dig += h;
-        c_vdig += mix64(h ^ vb);
+        h = item_hash(e, root, F, meta);
+        hv = mix64(h ^ vb);
       }
       uint64_t fbyte = (uint64_t)root * P.nfam + P.fam_slot;
       uint32_t fl = (vb & 8) ? CPL_RF_NOFIT_FLAG : 0;
@@ -357,15 +356,37 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
         split = true;
       } else {
         switch (v) {
-          case V_PROVEN: fl |= CPL_RF_PROVEN; c_cert++; break;
-          case V_EMPTY: fl |= CPL_RF_EMPTY; c_cert++; break;
-          case V_OOD: fl |= CPL_RF_OOD; c_cert++; break;
-          case V_CRITICAL: fl |= CPL_RF_CRITICAL; c_unres++; break;
-          case V_NOFIT: fl |= CPL_RF_NOFIT; c_unres++; break;
-          default: fl |= CPL_RF_TRAP; c_unres++; break;
+          case V_PROVEN: fl |= CPL_RF_PROVEN; cert = 1; break;
+          case V_EMPTY: fl |= CPL_RF_EMPTY; cert = 1; break;
+          case V_OOD: fl |= CPL_RF_OOD; cert = 1; break;
+          case V_CRITICAL: fl |= CPL_RF_CRITICAL; cert = 2; break;
+          case V_NOFIT: fl |= CPL_RF_NOFIT; cert = 2; break;
+          default: fl |= CPL_RF_TRAP; cert = 2; break;
         }
       }
       if (fl && P.root_flags) flag_root(P.root_flags, fbyte, fl);
+    }
+    {
+      const int v = vb & 7;
+      c_eval += __popcll(__ballot(valid));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c_v[k] += __popcll(__ballot(valid && v == k));
+      c_cert += __popcll(__ballot(cert == 1));
+      c_unres += __popcll(__ballot(cert == 2));
+      if (EXT) {
+        c_f32 += __popcll(__ballot(hit));
+        c_f32t += __popcll(__ballot(tried));
+      }
+      if (P.digest) {
+        uint64_t dg = h, dv = hv;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          dg += __shfl_xor(dg, off, 64);
+          dv += __shfl_xor(dv, off, 64);
+        }
+        c_dig += dg;
+        c_vdig += dv;
+      }
     }
     // ---- compaction of children: wave ballot + one atomic per wave ----
     uint64_t mask = __ballot(split);
@@ -432,8 +453,12 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
       }
     }
   }
-  const unsigned long long acc[16] = {c_eval, c_v[0], c_v[1], c_v[2], c_v[3], c_v[4], c_v[5], c_v[6],
-                                      c_v[7], c_dig, c_vdig, c_cert, c_unres, c_f32, c_f32t, 0};
+  // wave totals enter the block sum once (lane 0)
+  const bool l0 = lane == 0;
+  const unsigned long long acc[16] = {l0 ? c_eval : 0,  l0 ? c_v[0] : 0,  l0 ? c_v[1] : 0, l0 ? c_v[2] : 0,
+                                      l0 ? c_v[3] : 0,  l0 ? c_v[4] : 0,  l0 ? c_v[5] : 0, l0 ? c_v[6] : 0,
+                                      l0 ? c_v[7] : 0,  l0 ? c_dig : 0,   l0 ? c_vdig : 0, l0 ? c_cert : 0,
+                                      l0 ? c_unres : 0, l0 ? c_f32 : 0,   l0 ? c_f32t : 0, 0};
   block_stats_add(stat_slot(P, P.depth, F, blockIdx.x), acc);
 }
 
@@ -664,11 +689,12 @@ __global__ __launch_bounds__(256) void k_record(RefineParams P, Worklist in, uin
     const int depth = (int)(meta >> 16);
     const double lbrsp = item_lbrsp(P, meta, la_idx);
     const IV la(e[0], e[1]), r1(e[2], e[3]), r2(e[4], e[5]), p3(e[6], e[7]);
-    const int v = eval_synthetic<F>(la, r1, r2, p3, P.clip != 0, lbrsp) & 7;
+    const uint8_t vb = eval_synthetic<F>(la, r1, r2, p3, P.clip != 0, lbrsp);
+    const int v = vb & 7;
     if (v != V_PROVEN && v != V_CRITICAL && v != V_TRAP) continue;
     bool hit = false;
     (void)eval_synthetic<F, true>(la, r1, r2, p3, P.clip != 0, lbrsp, true, &hit);
-    float t[8] = {hit ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, (float)v, (float)depth};
+    float t[8] = {hit ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, (float)vb, (float)depth};
     if (v == V_CRITICAL && depth < P.max_depth && P.split_mode == 0) {
       t[5] = 1.0f;
       for (int a = 0; a < 4; ++a) {

@@ -260,7 +260,7 @@ int cpl_debug_prefilter(int family, const double* cells, size_t n, uint8_t* fp32
  * targets:
  * n x 8 floats (0: FP32 prefilter proves the cell, 1..4: fraction of the two
  * children settled when splitting axis a (mode-0 runs, CRITICAL cells above
- * max depth), 5: 1 where 1..4 are defined, 6: FP64 verdict, 7: depth).
+ * max depth), 5: 1 where 1..4 are defined, 6: FP64 verdict byte, 7: depth).
  * Runs the plain path; family_mask selects the families (records of one
  * family when one bit is set). */
 int cpl_mlp_collect(const cpl_refine_cfg* cfg, uint64_t max_records, uint32_t keep_every, float* features,

@@ -76,9 +76,9 @@ def test_collect_counts_match_the_run(cpl):
     reach = sum(st.verdicts[d][v] for d in range(st.rounds) for v in (abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP))
     assert feat.shape[0] == reach
     assert np.all(np.abs(feat) <= 1.0)
-    v = targ[:, 6].astype(int)
+    v = targ[:, 6].astype(int) & 7  # the verdict byte: code in bits 0-2
     assert set(np.unique(v)) <= {abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP}
-    assert np.all(targ[targ[:, 0] > 0, 6] == abi.V_PROVEN)  # FP32 never proves what FP64 does not
+    assert np.all(v[targ[:, 0] > 0] == abi.V_PROVEN)  # FP32 never proves what FP64 does not
     crit = (v == abi.V_CRITICAL) & (targ[:, 7] < cfg.max_depth)
     assert np.array_equal(targ[:, 5] > 0, crit)
     assert set(np.unique(targ[:, 1:5])) <= {0.0, 0.5, 1.0}
