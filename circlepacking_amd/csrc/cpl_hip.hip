@@ -141,7 +141,7 @@ __global__ void k_debug_geom(int fn, const double* in, double* out, int* trap, s
     case 3: r = collision_angle_outer_outer(ra, rb, t); break;
     case 4: r = semicircle_angle_outer(ra); break;
     case 5: r = semicircle_angle_inner(la, ra); break;
-    default: r = compute_collision_angle(la, ra, rb, t); break;
+    default: r = compute_collision_angle<3>(la, ra, rb, t); break;
   }
   out[2 * i] = r.lb;
   out[2 * i + 1] = r.ub;

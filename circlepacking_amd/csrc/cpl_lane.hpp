@@ -64,8 +64,21 @@ __device__ __forceinline__ UB is_obtuse(IVT<T> hyp, IVT<T> s1, IVT<T> s2) {
 }
 
 // collision_angles.cu:37-117
+//
+// W names the endpoints of the result the caller reads (bit 0: lb, bit 1: ub;
+// every caller on the path reads one).  An arcsine that reaches only an unread
+// endpoint is not evaluated; the quotients, their domain checks and therefore
+// every trap decision are those of the full computation, and when both branches
+// may apply the obtuse branch's result is computed in full, because its
+// emptiness decides between taking and joining the acute one.  Read endpoints
+// are bit-identical to the full computation's; unread ones are unspecified.
 template <class T>
+__device__ __forceinline__ bool asin_out_of_domain(IVT<T> x) {  // asin_iv's trap test
+  return x.lb < T(-1) || x.ub > T(1);
+}
+template <class T, int W>
 __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> lc, IVT<T> sc, IVT<T> ab) {
+  static_assert(W >= 1 && W <= 3, "endpoint selection");
   int trap = 0;
   using I = IVT<T>;
   lc.tighten_lb(sc.lb);
@@ -93,11 +106,16 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
       if (lquot.empty() || squot.empty()) {
         if (definitely(obt)) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
       } else {
-        I alpha = asin_iv(lquot, trap);
-        if (trap) return Angle<T>{result, trap};
-        I beta = asin_iv(squot, trap);
-        if (trap) return Angle<T>{result, trap};
-        result = alpha - beta;
+        if (asin_out_of_domain(lquot) || asin_out_of_domain(squot)) {
+          trap = T_ASIN;
+          return Angle<T>{result, trap};
+        }
+        // result = asin(lquot) - asin(squot), lower bound clamped at 0
+        const bool full = W == 3 || !definitely(obt);
+        T rl = T(0), ru = T(0);
+        if ((W & 1) || full) rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
+        if ((W & 2) || full) ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
+        result = I(rl, ru);
         result.tighten_lb(T(0));
       }
     }
@@ -110,21 +128,27 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
     if (sx.empty()) {
       if (!possibly(obt)) { trap = T_ACU_X; return Angle<T>{result, trap}; }
     } else {
-      I alpha = asin_iv(lx / lc, trap);
-      if (trap) return Angle<T>{result, trap};
-      I beta = asin_iv(sx / sc, trap);
-      if (trap) return Angle<T>{result, trap};
-      beta.tighten_lb(T(0));
-      I r = alpha + beta;
+      I lquot = lx / lc;
+      if (asin_out_of_domain(lquot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
+      I squot = sx / sc;
+      if (asin_out_of_domain(squot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
+      // r = asin(lquot) + asin(squot), the second's lower bound clamped at 0
+      I r(T(0), T(0));
+      if (W & 1) {
+        I beta(dasin_rd(squot.lb), T(0));
+        beta.tighten_lb(T(0));
+        r.lb = add_rd(dasin_rd(lquot.lb), beta.lb);
+      }
+      if (W & 2) r.ub = add_ru(dasin_ru(lquot.ub), dasin_ru(squot.ub));
       if (result.empty()) result = r;
       else result.do_join(r);
     }
   }
   return Angle<T>{result, trap};
 }
-template <class T>
+template <int W, class T>
 __device__ __forceinline__ IVT<T> compute_collision_angle(IVT<T> lc, IVT<T> sc, IVT<T> ab, int& trap) {
-  const Angle<T> a = compute_collision_angle_r(lc, sc, ab);
+  const Angle<T> a = compute_collision_angle_r<T, W>(lc, sc, ab);
   if (a.trap) trap = a.trap;
   return a.v;
 }
@@ -139,7 +163,7 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_inner_inner_r(IVT<
     I lc = la.lb + I(ra.ub, ra.ub);
     I sc = la.lb + I(rb.ub, rb.ub);
     I ab = I(ra.ub, ra.ub) + rb.ub;
-    ub = compute_collision_angle(lc, sc, ab, trap).ub;
+    ub = compute_collision_angle<2>(lc, sc, ab, trap).ub;
     if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   {
@@ -147,7 +171,7 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_inner_inner_r(IVT<
     I sc = la.ub + I(rb.lb, rb.lb);
     I ab = I(ra.lb, ra.lb) + rb.lb;
     if (ab.lb <= T(0)) lb = T(0);
-    else lb = compute_collision_angle(lc, sc, ab, trap).lb;
+    else lb = compute_collision_angle<1>(lc, sc, ab, trap).lb;
   }
   return Angle<T>{I(lb, ub), trap};
 }
@@ -160,14 +184,16 @@ __device__ __forceinline__ IVT<T> collision_angle_inner_inner(IVT<T> la, IVT<T> 
 
 // collision_angles.cu:144-159 (outer_inner: ca = 1 - ra, cb = la + rb) and
 // :176-191 (inner_outer: ca = la + ra, cb = 1 - rb).
-template <class T>
+// W: the endpoints the caller reads, as for compute_collision_angle_r (the
+// join combines like endpoints only).
+template <int W, class T>
 __device__ __forceinline__ IVT<T> do_collision_angle(IVT<T> ca, IVT<T> cb, IVT<T> ab, int& trap) {
   UB ca_longer = (cb <= ca);
-  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab, trap);
-  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab, trap);
-  IVT<T> result = compute_collision_angle(ca, cb, ab, trap);
+  if (definitely(ca_longer)) return compute_collision_angle<W>(ca, cb, ab, trap);
+  if (definitely(!ca_longer)) return compute_collision_angle<W>(cb, ca, ab, trap);
+  IVT<T> result = compute_collision_angle<W>(ca, cb, ab, trap);
   if (trap) return result;
-  result.do_join(compute_collision_angle(cb, ca, ab, trap));
+  result.do_join(compute_collision_angle<W>(cb, ca, ab, trap));
   return result;
 }
 
@@ -181,11 +207,11 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_outer_inner_r(IVT<
     lb = T(0);
   } else {
     I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
-    lb = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).lb;
+    lb = do_collision_angle<1>(T(1) - pa, la + pb, pa + pb, trap).lb;
     if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
-  ub = do_collision_angle(T(1) - pa, la + pb, pa + pb, trap).ub;
+  ub = do_collision_angle<2>(T(1) - pa, la + pb, pa + pb, trap).ub;
   return Angle<T>{I(lb, ub), trap};
 }
 template <class T>
@@ -205,11 +231,11 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_inner_outer_r(IVT<
     lb = T(0);
   } else {
     I pa(ra.lb, ra.lb), pb(rb.lb, rb.lb);
-    lb = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).lb;
+    lb = do_collision_angle<1>(la + pa, T(1) - pb, pa + pb, trap).lb;
     if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   I pa(ra.ub, ra.ub), pb(rb.ub, rb.ub);
-  ub = do_collision_angle(la + pa, T(1) - pb, pa + pb, trap).ub;
+  ub = do_collision_angle<2>(la + pa, T(1) - pb, pa + pb, trap).ub;
   return Angle<T>{I(lb, ub), trap};
 }
 template <class T>
@@ -229,7 +255,7 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_outer_outer_r(IVT<
     I lc = I(T(1), T(1)) - rb.ub;
     I sc = I(T(1), T(1)) - ra.ub;
     I ab = I(ra.ub, ra.ub) + rb.ub;
-    ub = compute_collision_angle(lc, sc, ab, trap).ub;
+    ub = compute_collision_angle<2>(lc, sc, ab, trap).ub;
     if (trap) return Angle<T>{I(T(0), T(0)), trap};
   }
   {
@@ -237,7 +263,7 @@ __device__ __attribute__((noinline)) Angle<T> collision_angle_outer_outer_r(IVT<
     I sc = I(T(1), T(1)) - ra.lb;
     I ab = I(ra.lb, ra.lb) + rb.lb;
     if (ab.lb <= T(0)) lb = T(0);
-    else lb = compute_collision_angle(lc, sc, ab, trap).lb;
+    else lb = compute_collision_angle<1>(lc, sc, ab, trap).lb;
   }
   return Angle<T>{I(lb, ub), trap};
 }
