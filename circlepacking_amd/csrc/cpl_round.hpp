@@ -20,10 +20,10 @@
 // add, sub and mul are single IEEE instructions, correctly rounded in the
 // selected mode.  Division and square root are multi-instruction sequences
 // whose directed-mode results are NOT correctly rounded, so they are computed
-// as: q = RN(a/b) by the compiler's IEEE sequence, then one fused
-// multiply-add pair executed in the target mode,
+// as: a faithful quotient q (div_faithful: one of the two doubles around a/b),
+// then one fused multiply-add pair executed in the target mode,
 //     r = fma(-q, b, a)          (the exact remainder, rounded toward the target)
-//     result = fma(r, y, q)      (y a power of two ~ 1/(2|b|))
+//     result = fma(r, y, q)      (y a power of two, |y b| < 1)
 // whose directed rounding lands on q or its neighbour exactly as the rounded
 // true quotient does (Markstein-style correction; see div_fast_ok below).
 #pragma once
@@ -152,29 +152,59 @@ __device__ __forceinline__ double div_slow(double a, double b) {
   return UP ? scale_ru(ds, pow2(shift)) : scale_rd(ds, pow2(shift));
 }
 
-// Fast-path condition: |q| and |b| in [2^-480, 2^480).  Then |a| = |q b|(1+e)
-// lies in [2^-961, 2^961], so the remainder a - q*b is exactly representable
-// and y = +-2^-600 satisfies 0 < y*b <= 2^-120 < 1: q + r*y is strictly between
-// q and the true quotient (the fma computes r*y exactly, however small).
+// Fast-path condition: |q| and |b| in [2^-480, 2^480), q faithful.  Then
+// |a| = |q b|(1+e) with |e| < 2^-51 lies in [2^-961, 2^961], so the remainder
+// a - q*b of the faithful q is exactly representable, and y = +-2^-600 (the
+// sign of b) satisfies 0 < y*b <= 2^-120 < 1: q + r*y is strictly between q
+// and the true quotient (the fma computes r*y exactly, however small), so it
+// rounds in the target direction to q when the quotient lies on the target
+// side of q and to q's neighbour on that side otherwise.
 __device__ __forceinline__ bool div_fast_ok(double q, double b) {
   double aq = __builtin_fabs(q), ab = __builtin_fabs(b);
   return (aq >= 0x1p-480) & (aq < 0x1p+480) & (ab >= 0x1p-480) & (ab < 0x1p+480);
 }
 __device__ __forceinline__ double div_y(double b) { return __builtin_copysign(0x1p-600, b); }
 
+// Quotient for the fast path: the hardware reciprocal estimate, two Newton
+// steps (r <- r + r (1 - b r)) and one residual correction
+// q1 = q0 + (a - b q0) r, all in round-to-nearest.  The correction squares the
+// reciprocal's relative error, so q1 is faithful (one of the two doubles
+// around a/b) whenever q1 and b pass div_fast_ok, for any reciprocal estimate
+// better than about 2^-7; the fix step below needs no more than that, since
+// the remainder of a faithful quotient is exactly representable.  Outside the
+// fast range the result is not used.
+__device__ __forceinline__ double div_faithful(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q0 = a * r;
+  const double rem = __builtin_fma(-b, q0, a);
+  return __builtin_fma(rem, r, q0);
+}
+
+// Quotients the fast path does not take: 0/b is the exact signed zero for a
+// finite nonzero b (IEEE division otherwise: NaN for 0/0, signed zero for
+// 0/inf); every other operand pair goes through the scaled slow path.
+template <bool UP>
+__device__ __forceinline__ double div_rare(double a, double b) {
+  if (a == 0.0) return (__builtin_isfinite(b) && b != 0.0) ? a * b : a / b;
+  return div_slow<UP>(a, b);
+}
+
 template <bool UP>
 __device__ __forceinline__ double div_dir(double a, double b) {
-  double q = a / b;  // IEEE round-to-nearest
+  double q = div_faithful(a, b);
   if (__builtin_expect(div_fast_ok(q, b), 1)) return UP ? div_fix_ru(q, b, a, div_y(b)) : div_fix_rd(q, b, a, div_y(b));
-  if (a == 0.0) return q;  // 0/b: the exact signed zero (or NaN for 0/0)
-  return div_slow<UP>(a, b);
+  return div_rare<UP>(a, b);
 }
 __device__ __forceinline__ double div_rd(double a, double b) { return div_dir<false>(a, b); }
 __device__ __forceinline__ double div_ru(double a, double b) { return div_dir<true>(a, b); }
 
 // [rd(al / bl), ru(au / bu)]: both corrections in one statement.
 __device__ __forceinline__ void div_pair(double al, double bl, double au, double bu, double& l, double& u) {
-  double ql = al / bl, qu = au / bu;
+  double ql = div_faithful(al, bl), qu = div_faithful(au, bu);
   if (__builtin_expect(div_fast_ok(ql, bl) & div_fast_ok(qu, bu), 1)) {
     double yl = div_y(bl), yu = div_y(bu), rl, ru;
     asm("s_nop 1\n\t" CPL_MODE_RD "v_fma_f64 %2, -%4, %5, %6\n\t"
@@ -184,8 +214,8 @@ __device__ __forceinline__ void div_pair(double al, double bl, double au, double
         : "v"(ql), "v"(bl), "v"(al), "v"(yl), "v"(qu), "v"(bu), "v"(au), "v"(yu));
     return;
   }
-  l = div_fast_ok(ql, bl) ? div_fix_rd(ql, bl, al, div_y(bl)) : (al == 0.0 ? ql : div_slow<false>(al, bl));
-  u = div_fast_ok(qu, bu) ? div_fix_ru(qu, bu, au, div_y(bu)) : (au == 0.0 ? qu : div_slow<true>(au, bu));
+  l = div_fast_ok(ql, bl) ? div_fix_rd(ql, bl, al, div_y(bl)) : div_rare<false>(al, bl);
+  u = div_fast_ok(qu, bu) ? div_fix_ru(qu, bu, au, div_y(bu)) : div_rare<true>(au, bu);
 }
 
 // ---- square root ------------------------------------------------------------
