@@ -367,6 +367,12 @@ __global__ __launch_bounds__(LB, 4) void k_round(RefineParams P, Worklist in, ui
       }
       if (fl && P.root_flags) flag_root(P.root_flags, fbyte, fl);
     }
+    // ---- compaction of children: wave ballot + one atomic per wave, issued
+    // before the statistics so its round trip overlaps them ----
+    const uint64_t mask = __ballot(split);
+    const int leader = mask ? __ffsll((unsigned long long)mask) - 1 : 0;
+    uint32_t basepos = 0;
+    if (mask && lane == leader) basepos = atomicAdd(P.out_count, (uint32_t)__popcll(mask) * (uint32_t)P.n_children);
     {
       const int v = vb & 7;
       c_eval += __popcll(__ballot(valid));
@@ -389,14 +395,8 @@ __global__ __launch_bounds__(LB, 4) void k_round(RefineParams P, Worklist in, ui
         c_vdig += dv;
       }
     }
-    // ---- compaction of children: wave ballot + one atomic per wave ----
-    uint64_t mask = __ballot(split);
     if (mask) {
-      uint32_t cnt = (uint32_t)__popcll(mask);
       uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      int leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t basepos = 0;
-      if (lane == leader) basepos = atomicAdd(P.out_count, cnt * (uint32_t)P.n_children);
       basepos = __shfl(basepos, leader, 64);
       if (split) {
         uint64_t pos = (uint64_t)basepos + (uint64_t)rank * P.n_children;

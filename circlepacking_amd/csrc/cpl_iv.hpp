@@ -13,6 +13,10 @@
 
 namespace cpl {
 
+// True when c holds on every active lane of the wave (a wave-uniform branch
+// condition for the sign-case fast paths below; inactive lanes do not vote).
+__device__ __forceinline__ bool all_lanes(bool c) { return __ballot(!c) == 0; }
+
 struct UB {
   bool lb, ub;  // lb: definitely, ub: possibly (interval.hpp:32-82)
   __device__ UB operator!() const { return UB{!ub, !lb}; }
@@ -42,8 +46,13 @@ struct IVT {
     return IVT(l, u);
   }
   __device__ IVT square() const {  // interval.cuh:352-355
-    IVT a = abs();
     IVT r;
+    if (all_lanes(lb >= T(0) && ub > T(0))) {
+      // abs() is the identity on such an interval (a -0 lower bound included)
+      mul_pair(lb, lb, ub, ub, r.lb, r.ub);
+      return r;
+    }
+    IVT a = abs();
     mul_pair(a.lb, a.lb, a.ub, a.ub, r.lb, r.ub);
     return r;
   }
@@ -104,6 +113,12 @@ __device__ __forceinline__ IVT<T> operator*(const IVT<T>& a, const IVT<T>& b) {
 // contains zero gives (-inf, +inf) as in the reference.
 template <class T>
 __device__ __forceinline__ IVT<T> operator/(const IVT<T>& a, const IVT<T>& o) {
+  if (all_lanes(o.lb > T(0) && a.lb >= T(0))) {
+    // the case the sign selection below picks for a >= 0 over a positive divisor
+    IVT<T> r;
+    div_pair(a.lb, o.ub, a.ub, o.lb, r.lb, r.ub);
+    return r;
+  }
   const bool pos = o.lb > 0, neg = o.ub < 0;
   const bool anneg = a.lb >= 0, anpos = a.ub <= 0;  // a >= 0, a <= 0
   T nl, dl, nu, du;
