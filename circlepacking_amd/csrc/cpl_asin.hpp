@@ -14,20 +14,21 @@
 
 namespace cpl {
 
+// Horner steps as fma_sc (coefficient in scalar registers, see cpl_round.hpp).
 __device__ __forceinline__ double asin_poly_eval(double z) {
   double p = 0x1.d72b2bc8155f8p-6;
-  p = __builtin_fma(p, z, -0x1.e6aaa8a0a04ccp-7);
-  p = __builtin_fma(p, z, 0x1.1d189408314eep-6);
-  p = __builtin_fma(p, z, 0x1.65a9c4dfcf8b2p-8);
-  p = __builtin_fma(p, z, 0x1.52420b04b37bep-7);
-  p = __builtin_fma(p, z, 0x1.782651caa6547p-7);
-  p = __builtin_fma(p, z, 0x1.c9cf07674736ap-7);
-  p = __builtin_fma(p, z, 0x1.1c4d35cf95421p-6);
-  p = __builtin_fma(p, z, 0x1.6e8bb1c8209a2p-6);
-  p = __builtin_fma(p, z, 0x1.f1c71c1db0623p-6);
-  p = __builtin_fma(p, z, 0x1.6db6db6e31f13p-5);
-  p = __builtin_fma(p, z, 0x1.3333333332ecap-4);
-  p = __builtin_fma(p, z, 0x1.5555555555556p-3);
+  p = fma_sc(p, z, -0x1.e6aaa8a0a04ccp-7);
+  p = fma_sc(p, z, 0x1.1d189408314eep-6);
+  p = fma_sc(p, z, 0x1.65a9c4dfcf8b2p-8);
+  p = fma_sc(p, z, 0x1.52420b04b37bep-7);
+  p = fma_sc(p, z, 0x1.782651caa6547p-7);
+  p = fma_sc(p, z, 0x1.c9cf07674736ap-7);
+  p = fma_sc(p, z, 0x1.1c4d35cf95421p-6);
+  p = fma_sc(p, z, 0x1.6e8bb1c8209a2p-6);
+  p = fma_sc(p, z, 0x1.f1c71c1db0623p-6);
+  p = fma_sc(p, z, 0x1.6db6db6e31f13p-5);
+  p = fma_sc(p, z, 0x1.3333333332ecap-4);
+  p = fma_sc(p, z, 0x1.5555555555556p-3);
   return p;
 }
 

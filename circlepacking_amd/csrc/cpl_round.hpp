@@ -86,6 +86,17 @@ __device__ __forceinline__ void mul_pair(double al, double bl, double au, double
       : "v"(al), "v"(bl), "v"(au), "v"(bu));
 }
 
+// ---- fma with a scalar-register addend --------------------------------------
+// fma(a, b, c) in round-to-nearest (the ambient mode) as the three-operand
+// v_fma_f64 with c in an SGPR pair.  For a constant c the compiler otherwise
+// picks the two-address v_fmac_f64, whose accumulator must be a VGPR pair:
+// two v_mov per Horner step (the 64-bit constant cannot be an operand literal).
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+}
+
 // ---- bit helpers ------------------------------------------------------------
 __device__ __forceinline__ uint64_t dbits(double d) { return __double_as_longlong(d); }
 __device__ __forceinline__ double dfrom(uint64_t b) { return __longlong_as_double((long long)b); }
