@@ -287,11 +287,10 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
 // LB: launch bound (maximum threads per block) the register allocation is made for.
 // EXT: extended kernels with the FP32 prefilter (enabled per depth by
 // P.prefilter_depths) and the learned proposal (P.mlp).
-// The second launch bound asks for at least 4 waves per SIMD (<= 128 VGPRs):
-// the kernel is latency bound; with the criteria out of line the kernel body
-// fits and the extra waves outweigh the call's register saves.
+// The second launch bound asks for at least 3 waves per SIMD (<= 168 VGPRs):
+// the kernel is latency bound and the extra wave outweighs a few spills.
 template <int F, bool ROOT, int LB = 256, bool EXT = false>
-__global__ __launch_bounds__(LB, 4) void k_round(RefineParams P, Worklist in, uint64_t n) {
+__global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, uint64_t n) {
   if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
   if (P.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *P.zero_count = 0u;
   const int lane = threadIdx.x & 63;
