@@ -516,7 +516,14 @@ static inline IV get_subinterval(IV iv, int i, int n) {
 static inline UB is_obtuse(IV hypot, IV s1, IV s2) { return hypot.square() > s1.square() + s2.square(); }
 
 // collision_angles.cu:37-117
-static inline IV compute_collision_angle(IV lc, IV sc, IV ab) {
+//
+// `reads` (bit 0: lb, bit 1: ub) names the endpoints of the result the caller
+// uses; it changes no value, only the flop count of the counting build: an
+// arcsine whose result reaches only an unused endpoint is not counted (the
+// product skips it; when both branches may apply, the first branch's result is
+// needed in full and everything is counted).
+static inline long long asin_flops(double x) { return std::fabs(x) <= 0.5 ? 28 : 42; }
+static inline IV compute_collision_angle(IV lc, IV sc, IV ab, int reads = 3) {
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
   IV result(0.0, -1.0);
@@ -544,6 +551,10 @@ static inline IV compute_collision_angle(IV lc, IV sc, IV ab) {
       } else {
         IV alpha = asin(lquot);
         IV beta = asin(squot);
+        if (reads != 3 && definitely(obt)) {
+          if (!(reads & 1)) CPLO_FLOPS(-(asin_flops(lquot.lb) + asin_flops(squot.ub)));
+          if (!(reads & 2)) CPLO_FLOPS(-(asin_flops(lquot.ub) + asin_flops(squot.lb)));
+        }
         result = alpha - beta;
         result.tighten_lb(0.0);
       }
@@ -557,8 +568,12 @@ static inline IV compute_collision_angle(IV lc, IV sc, IV ab) {
     if (sx.empty()) {
       if (!possibly(obt)) throw Trap{TRAP_ACUTE_EMPTY_X};
     } else {
-      IV alpha = asin(lx / lc);
-      IV beta = asin(sx / sc);
+      IV lquot = lx / lc;
+      IV alpha = asin(lquot);
+      IV squot = sx / sc;
+      IV beta = asin(squot);
+      if (!(reads & 1)) CPLO_FLOPS(-(asin_flops(lquot.lb) + asin_flops(squot.lb)));
+      if (!(reads & 2)) CPLO_FLOPS(-(asin_flops(lquot.ub) + asin_flops(squot.ub)));
       beta.tighten_lb(0.0);
       IV r = alpha + beta;
       if (result.empty()) result = r;
@@ -575,28 +590,28 @@ static inline IV collision_angle_inner_inner(IV la, IV ra, IV rb) {
     IV lc = la.lb + IV(ra.ub, ra.ub);
     IV sc = la.lb + IV(rb.ub, rb.ub);
     IV ab = IV(ra.ub, ra.ub) + rb.ub;
-    ub = compute_collision_angle(lc, sc, ab).ub;
+    ub = compute_collision_angle(lc, sc, ab, 2).ub;
   }
   {
     IV lc = la.ub + IV(ra.lb, ra.lb);
     IV sc = la.ub + IV(rb.lb, rb.lb);
     IV ab = IV(ra.lb, ra.lb) + rb.lb;
     if (ab.lb <= 0.0) lb = 0.0;
-    else lb = compute_collision_angle(lc, sc, ab).lb;
+    else lb = compute_collision_angle(lc, sc, ab, 1).lb;
   }
   return IV(lb, ub);
 }
 
 // collision_angles.cu:144-159
-static inline IV do_collision_angle_outer_inner(IV la, IV ra, IV rb) {
+static inline IV do_collision_angle_outer_inner(IV la, IV ra, IV rb, int reads) {
   IV ca = 1.0 - ra;
   IV cb = la + rb;
   IV ab = ra + rb;
   UB ca_longer = (cb <= ca);
-  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab);
-  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab);
-  IV result = compute_collision_angle(ca, cb, ab);
-  result.do_join(compute_collision_angle(cb, ca, ab));
+  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab, reads);
+  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab, reads);
+  IV result = compute_collision_angle(ca, cb, ab, reads);
+  result.do_join(compute_collision_angle(cb, ca, ab, reads));
   return result;
 }
 
@@ -604,21 +619,21 @@ static inline IV do_collision_angle_outer_inner(IV la, IV ra, IV rb) {
 static inline IV collision_angle_outer_inner(IV la, IV ra, IV rb) {
   double lb, ub;
   if (possibly(2.0 * (ra + rb) <= 1.0 - la)) lb = 0.0;
-  else lb = do_collision_angle_outer_inner(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb)).lb;
-  ub = do_collision_angle_outer_inner(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub)).ub;
+  else lb = do_collision_angle_outer_inner(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb), 1).lb;
+  ub = do_collision_angle_outer_inner(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub), 2).ub;
   return IV(lb, ub);
 }
 
 // collision_angles.cu:176-191
-static inline IV do_collision_angle_inner_outer(IV la, IV ra, IV rb) {
+static inline IV do_collision_angle_inner_outer(IV la, IV ra, IV rb, int reads) {
   IV ca = la + ra;
   IV cb = 1.0 - rb;
   IV ab = ra + rb;
   UB ca_longer = (cb <= ca);
-  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab);
-  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab);
-  IV result = compute_collision_angle(ca, cb, ab);
-  result.do_join(compute_collision_angle(cb, ca, ab));
+  if (definitely(ca_longer)) return compute_collision_angle(ca, cb, ab, reads);
+  if (definitely(!ca_longer)) return compute_collision_angle(cb, ca, ab, reads);
+  IV result = compute_collision_angle(ca, cb, ab, reads);
+  result.do_join(compute_collision_angle(cb, ca, ab, reads));
   return result;
 }
 
@@ -626,8 +641,8 @@ static inline IV do_collision_angle_inner_outer(IV la, IV ra, IV rb) {
 static inline IV collision_angle_inner_outer(IV la, IV ra, IV rb) {
   double lb, ub;
   if (possibly(2.0 * (ra + rb) <= 1.0 - la)) lb = 0.0;
-  else lb = do_collision_angle_inner_outer(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb)).lb;
-  ub = do_collision_angle_inner_outer(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub)).ub;
+  else lb = do_collision_angle_inner_outer(la, IV(ra.lb, ra.lb), IV(rb.lb, rb.lb), 1).lb;
+  ub = do_collision_angle_inner_outer(la, IV(ra.ub, ra.ub), IV(rb.ub, rb.ub), 2).ub;
   return IV(lb, ub);
 }
 
@@ -638,14 +653,14 @@ static inline IV collision_angle_outer_outer(IV ra, IV rb) {
     IV lc = IV(1.0, 1.0) - rb.ub;
     IV sc = IV(1.0, 1.0) - ra.ub;
     IV ab = IV(ra.ub, ra.ub) + rb.ub;
-    ub = compute_collision_angle(lc, sc, ab).ub;
+    ub = compute_collision_angle(lc, sc, ab, 2).ub;
   }
   {
     IV lc = IV(1.0, 1.0) - rb.lb;
     IV sc = IV(1.0, 1.0) - ra.lb;
     IV ab = IV(ra.lb, ra.lb) + rb.lb;
     if (ab.lb <= 0.0) lb = 0.0;
-    else lb = compute_collision_angle(lc, sc, ab).lb;
+    else lb = compute_collision_angle(lc, sc, ab, 1).lb;
   }
   return IV(lb, ub);
 }
