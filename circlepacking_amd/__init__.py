@@ -76,7 +76,7 @@ def prim(op, a, b=None):
 
 
 GEOM = {"inner_inner": 0, "outer_inner": 1, "inner_outer": 2, "outer_outer": 3,
-        "semicircle_outer": 4, "semicircle_inner": 5, "compute": 6}
+        "semicircle_outer": 4, "semicircle_inner": 5, "compute": 6, "compute_lb": 7, "compute_ub": 8}
 
 
 def debug_geom(fn, recs, fp32=False):

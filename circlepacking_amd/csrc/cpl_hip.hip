@@ -118,7 +118,8 @@ __global__ void k_interval_op(int op, const IV* a, const IV* b, IV* out, int* tr
 // FP64 (T = double) or in the FP32 prefilter precision (T = float, inputs
 // converted outward).  fn: 0 inner_inner 1 outer_inner 2 inner_outer
 // 3 outer_outer(ra, rb) 4 semicircle_outer(ra) 5 semicircle_inner(la, ra)
-// 6 compute_collision_angle(lc = la, sc = ra, ab = rb).
+// 6 compute_collision_angle(lc = la, sc = ra, ab = rb), 7 / 8 the same with
+// only the lower / upper endpoint computed.
 template <class T>
 __device__ __forceinline__ IVT<T> conv(IV x);
 template <>
@@ -141,7 +142,9 @@ __global__ void k_debug_geom(int fn, const double* in, double* out, int* trap, s
     case 3: r = collision_angle_outer_outer(ra, rb, t); break;
     case 4: r = semicircle_angle_outer(ra); break;
     case 5: r = semicircle_angle_inner(la, ra); break;
-    default: r = compute_collision_angle<3>(la, ra, rb, t); break;
+    case 6: r = compute_collision_angle<3>(la, ra, rb, t); break;
+    case 7: r = compute_collision_angle<1>(la, ra, rb, t); break;
+    default: r = compute_collision_angle<2>(la, ra, rb, t); break;
   }
   out[2 * i] = r.lb;
   out[2 * i + 1] = r.ub;
@@ -1240,7 +1243,7 @@ int cpl_eval_kernel_body(int family, const double* cells, size_t n, uint8_t* ver
 int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (fn < 0 || fn > 6 || !in || !out || !trap) return fail(CPL_ERR_ARG, "cpl_debug_geom: bad function or pointer");
+  if (fn < 0 || fn > 8 || !in || !out || !trap) return fail(CPL_ERR_ARG, "cpl_debug_geom: bad function or pointer");
   if (n == 0) return CPL_OK;
   if ((rc = g.scratch_in.ensure(6 * n * sizeof(double))) || (rc = g.scratch_out.ensure(2 * n * sizeof(double))) ||
       (rc = g.scratch_in2.ensure(n * sizeof(int))))

@@ -232,7 +232,9 @@ int cpl_interval_op(int op, const cpl_iv* a, const cpl_iv* b, cpl_iv* out, int* 
 /* Geometry of the criteria on n records of 6 doubles (la, ra, rb intervals):
  * fn 0 collision_angle_inner_inner 1 outer_inner 2 inner_outer (collision_angles.cu:119-205)
  * 3 outer_outer(ra, rb) (:207-231) 4 semicircle_angle_outer(ra) (:233-238)
- * 5 semicircle_angle_inner(la, ra) (:240-246) 6 compute_collision_angle(la, ra, rb) (:37-117).
+ * 5 semicircle_angle_inner(la, ra) (:240-246) 6 compute_collision_angle(la, ra, rb) (:37-117);
+ * 7 / 8 the same with only the lower / upper endpoint computed (the forms the
+ * criteria call; the other endpoint of out is unspecified).
  * fp32 = 1 evaluates in the prefilter precision on the outward FP32 enclosure
  * of the inputs.  out: n x (lb, ub); trap: n trap codes (0: none). */
 int cpl_debug_geom(int fn, int fp32, const double* in, double* out, int* trap, size_t n);
