@@ -16,7 +16,8 @@
 //                     primitives, geometry and FP32-prefilter results for the
 //                     parity tests
 // EXT template variants run the FP32 outward-rounded prefilter before the FP64
-// criteria and the learned proposal (BF16 MFMA) on each wave of cells.
+// criteria; k_propose writes the learned proposal (BF16 MFMA) of each round's
+// cells ahead of the round.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -289,7 +290,8 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
 // else items come from `in` (n items).
 // LB: launch bound (maximum threads per block) the register allocation is made for.
 // EXT: extended kernels with the FP32 prefilter (enabled per depth by
-// P.prefilter_depths) and the learned proposal (P.mlp).
+// P.prefilter_depths, gated by the proposal byte).  The proposal byte itself
+// (P.prop, written by k_propose) is read by every refinement kernel.
 // The second launch bound asks for at least 3 waves per SIMD (<= 168 VGPRs):
 // the kernel is latency bound and the extra wave outweighs a few spills.
 template <int F, bool ROOT, int LB = 256, bool EXT = false>
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(LB, 3) void k_round(RefineParams P, Worklist in, ui
       }
     }
     uint8_t pb = kNoProposal;
-    if (EXT && !ROOT && P.prop && valid) pb = P.prop[i];
+    if (!ROOT && P.prop && valid) pb = P.prop[i];
     if (valid) {
       int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
@@ -1319,7 +1321,7 @@ struct Plan {
   uint64_t cap = 0;
   bool digest = true;
   bool flags = false;
-  bool ext = false;       // extended kernels (prefilter and / or learned proposal)
+  bool ext = false;       // extended kernels (FP32 prefilter)
   bool learned = false;
   uint32_t prefilter_depths = 0;
   int block = 256;
@@ -1350,7 +1352,7 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
   pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT)) != 0;
-  pl.ext = (cfg->options & CPL_OPT_FP32) != 0 || pl.learned;
+  pl.ext = (cfg->options & CPL_OPT_FP32) != 0;
   pl.prefilter_depths = !(cfg->options & CPL_OPT_FP32) ? 0u : cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
   if (pl.learned)
     for (int f : pl.fams)
