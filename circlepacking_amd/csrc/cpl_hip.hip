@@ -557,6 +557,21 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
   block_stats_add(stat_slot(P, 0, F, blockIdx.x), acc);
 }
 
+// Zero family F's statistics records (every round and slot) and the five
+// counters and the overflow word of family slot `slot`: the head of that
+// family's captured chain in the graph form, so the chains need no common
+// prologue (and no cross-stream wait) before they start.
+__global__ void k_clear_family(RoundStats* stats, int F, uint32_t* counters, int slot) {
+  constexpr int W = sizeof(RoundStats) / sizeof(unsigned long long);
+  constexpr int per_round = STAT_SLOTS * W;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < CPL_MAX_ROUNDS * per_round) {
+    const int d = i / per_round, k = i % per_round;
+    reinterpret_cast<unsigned long long*>(stats + (size_t)(d * CPL_NUM_FAMILIES + F) * STAT_SLOTS)[k] = 0ull;
+  }
+  if (i < 6) counters[i * CPL_NUM_FAMILIES + slot] = 0u;  // counter buffers 0..4, overflow word (5)
+}
+
 // Sum the STAT_SLOTS copies of every (round, family) record into out.
 __global__ void k_reduce_stats(const RoundStats* __restrict__ in, RoundStats* __restrict__ out) {
   constexpr int W = sizeof(RoundStats) / sizeof(unsigned long long);
@@ -1370,11 +1385,13 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
 constexpr size_t kCounterBytes = 6 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64;
 
 // Device buffers: worklists [0], [1] (ping-pong) and [2] (live root items),
-// each with one region per family; counters[3][families] + overflow word.
+// each with one region per family; counters[5][family slots] + one overflow
+// word per family slot; one la table per family slot (the graph form's chains
+// each compute their own; the other forms use slot 0's).
 int ensure_buffers(const Plan& pl) {
   int rc;
   const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
-  if ((rc = g.la_table.ensure(pl.la_entries * 3 * sizeof(double)))) return rc;
+  if ((rc = g.la_table.ensure(pl.la_entries * 3 * sizeof(double) * CPL_NUM_FAMILIES))) return rc;
   if ((rc = g.root_flags.ensure(pl.flag_words * 4 + 4))) return rc;
   if ((rc = g.stats.ensure(pl.stats_bytes))) return rc;
   if ((rc = g.stats_sum.ensure(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats)))) return rc;
@@ -1387,7 +1404,8 @@ int ensure_buffers(const Plan& pl) {
 }
 
 uint32_t* counter(int buf, int slot) { return g.counters.as<uint32_t>() + buf * CPL_NUM_FAMILIES + slot; }
-uint32_t* overflow_word() { return g.counters.as<uint32_t>() + 5 * CPL_NUM_FAMILIES; }
+uint32_t* overflow_word(int slot = 0) { return g.counters.as<uint32_t>() + 5 * CPL_NUM_FAMILIES + slot; }
+double* la_copy(const Plan& pl, int slot) { return g.la_table.as<double>() + (size_t)slot * pl.la_entries * 3; }
 // Round counters of the graph path: round d reads rot_counter(d) and writes
 // rot_counter(d + 1), and clears rot_counter(d + 2) for the next round (it was
 // the input count of round d - 1), so no clearing node runs between rounds.
@@ -1467,8 +1485,11 @@ int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   std::memcpy(launch, stats->launch_seconds, sizeof(launch));
   std::memset(stats, 0, sizeof(*stats));
   std::memcpy(stats->launch_seconds, launch, sizeof(launch));
+  bool planned[CPL_NUM_FAMILIES] = {};
+  for (int f : pl.fams) planned[f] = true;
   for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
     for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
+      if (!planned[f]) continue;  // records of other families are not cleared by every form
       const RoundStats& r = rs[d * CPL_NUM_FAMILIES + f];
       stats->evaluations[d] += r.evaluations;
       for (int v = 0; v < 8; ++v) {
@@ -1631,15 +1652,12 @@ unsigned grid_tuned(uint64_t expect, const Plan& pl) {
   return (unsigned)std::max<uint64_t>(1, std::min(want, cap));
 }
 
-// Work shared by every family (cleared statistics, counters and flags, the la
-// table), enqueued on the library stream.
-void enqueue_prologue(const cpl_refine_cfg* cfg, const Plan& pl) {
+// Work shared by the families: clearing the per-root flags (their bytes
+// interleave the families), enqueued on the library stream.  Only captured and
+// replayed when flags are requested; then the family chains wait for it.
+void enqueue_prologue(const Plan& pl) {
   g_launch_stream = nullptr;
-  fill(g.stats.p, pl.stats_bytes / 4, 0u);
-  fill(g.counters.p, 5 * CPL_NUM_FAMILIES + 1, 0u);
-  if (pl.flags) fill(g.root_flags.p, pl.flag_words, 0u);
-  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, g.stream, cfg->box_lb[0],
-                     cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
+  fill(g.root_flags.p, pl.flag_words, 0u);
 }
 
 // Round d of family slot s with device-side sizes, enqueued on that family's
@@ -1653,6 +1671,8 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
   };
   g_launch_stream = fam_stream_of(s);
   set_family(P, pl, s);
+  P.la_table = la_copy(pl, s);
+  P.overflow = overflow_word(s);
   P.depth = d;
   if (d == 0) {
     P.in_count = nullptr;
@@ -1690,6 +1710,13 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
 }
 
 void enqueue_family(const cpl_refine_cfg* cfg, const Plan& pl, int s, const std::vector<uint32_t>& expect) {
+  g_launch_stream = fam_stream_of(s);
+  const int words = CPL_MAX_ROUNDS * STAT_SLOTS * (int)(sizeof(RoundStats) / sizeof(unsigned long long));
+  hipLaunchKernelGGL(k_clear_family, dim3((words + 255) / 256), dim3(256), 0, launch_stream(), g.stats.as<RoundStats>(),
+                     pl.fams[s], g.counters.as<uint32_t>(), s);
+  hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, launch_stream(),
+                     cfg->box_lb[0], cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, la_copy(pl, s));
+  g_launch_stream = nullptr;
   for (int d = 0; d <= cfg->max_depth; ++d) enqueue_round(cfg, pl, s, d, expect);
 }
 
@@ -1699,8 +1726,9 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
     g_graph.reset();
     for (int k = 0; k <= pl.nf; ++k) {
       hipStream_t cs = k == 0 ? g.stream : fam_stream_of(k - 1);
+      if (k == 0 && !pl.flags) continue;  // no shared prologue
       HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      if (k == 0) enqueue_prologue(cfg, pl);
+      if (k == 0) enqueue_prologue(pl);
       else enqueue_family(cfg, pl, k - 1, g_graph.expect);
       hipError_t e = hipStreamEndCapture(cs, &g_graph.graph[k]);
       if (e != hipSuccess) return fail(CPL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1715,8 +1743,11 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(hipEventRecord(ev0, g.stream));
-  HIP_TRY(hipGraphLaunch(g_graph.exec[0], g.stream));
-  HIP_TRY(hipEventRecord(g.fork_ev, g.stream));
+  const bool prologue = g_graph.exec[0] != nullptr;
+  if (prologue) {
+    HIP_TRY(hipGraphLaunch(g_graph.exec[0], g.stream));
+    HIP_TRY(hipEventRecord(g.fork_ev, g.stream));
+  }
   // submit the family chains with the most work (previous replay) first: host
   // submission of a captured chain takes time, the largest chain bounds the run
   std::vector<int> order(pl.nf);
@@ -1734,15 +1765,17 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
       HIP_TRY(hipGraphLaunch(g_graph.exec[1], g.stream));
       continue;
     }
-    HIP_TRY(hipStreamWaitEvent(fam_stream_of(s), g.fork_ev, 0));
+    if (prologue) HIP_TRY(hipStreamWaitEvent(fam_stream_of(s), g.fork_ev, 0));
     HIP_TRY(hipGraphLaunch(g_graph.exec[1 + s], fam_stream_of(s)));
     HIP_TRY(hipEventRecord(g.join_ev[s], fam_stream_of(s)));
   }
   for (int s = 1; s < pl.nf; ++s) HIP_TRY(hipStreamWaitEvent(g.stream, g.join_ev[s], 0));
   HIP_TRY(hipEventRecord(ev1, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
+  uint32_t ovfw[CPL_NUM_FAMILIES] = {};
+  HIP_TRY(hipMemcpy(ovfw, overflow_word(0), pl.nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
   uint32_t ovf = 0;
-  HIP_TRY(hipMemcpy(&ovf, overflow_word(), sizeof(ovf), hipMemcpyDeviceToHost));
+  for (int s = 0; s < pl.nf; ++s) ovf |= ovfw[s];
   std::memset(stats, 0, sizeof(*stats));
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
