@@ -120,6 +120,16 @@ def test_learned_split_runs_and_accounts(cpl, trained):
     assert np.all(f != 0)
 
 
+def test_learned_split_leaves_fewer_unresolved_cells(cpl, trained):
+    # the split-axis proposal's gain at equal depth on this grid (measured:
+    # evaluations -7 %, unresolved leaves -20 % with the 24^4-trained model)
+    base = _small()
+    plain, _ = cpl.refine_run(base)
+    s, _ = cpl.refine_run(abi.with_options(base, abi.OPT_LEARNED_SPLIT))
+    assert s.unresolved < 0.9 * plain.unresolved, (s.unresolved, plain.unresolved)
+    assert s.total_evaluations < plain.total_evaluations
+
+
 def test_learned_rejected_with_persistent_form(cpl, trained):
     with pytest.raises(cpl.CplError):
         cpl.refine_run(abi.with_options(_small(), abi.OPT_PERSISTENT | abi.OPT_HIER_ROOTS | abi.OPT_LEARNED))
