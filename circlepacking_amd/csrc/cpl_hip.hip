@@ -236,16 +236,18 @@ __device__ __forceinline__ int widest_axis(const RefineParams& P, uint32_t meta)
   return best;
 }
 
-// Learned proposal of one wave (all 64 lanes call it): outputs 0..4 of each
-// lane's cell.  It runs in a kernel of its own (k_propose) ahead of each round,
-// so that the evaluation kernels keep their register budget.
-__device__ __forceinline__ void propose(const RefineParams& P, bool valid, const double* e, float* y) {
+// Learned proposal of one wave (all 64 lanes call it) with the network's
+// operands already in registers: outputs 0..4 of each lane's cell.  It runs in a
+// kernel of its own (k_propose) ahead of each round, so that the evaluation
+// kernels keep their register budget.
+__device__ __forceinline__ void propose(const RefineParams& P, const MlpRegs& W, bool valid, const double* e,
+                                        float* y) {
   float f[8];
   if (valid) mlp_features(e, P.box_lb, P.box_ub, f);
   else
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] = 0.0f;
-  mlp_infer_wave(P.mlp, f, y);
+  mlp_infer_wave(W, f, y);
 }
 
 // Proposal byte of a cell: bit 0 gate (run the FP32 prefilter), bits 1-2 the
@@ -267,11 +269,17 @@ __device__ __forceinline__ int split_axis(const RefineParams& P, uint32_t meta, 
 }
 
 // Proposal bytes of a round's input items (before the round runs on them).
+// Each wave loads the network's operands into registers once and then walks
+// 64-cell groups with a wave-level grid stride (the operands are ~23 KB per
+// wave; reloading them per group dominated the kernel's time).
 __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, uint64_t n, uint8_t* out) {
   if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const uint64_t i = base + threadIdx.x;
+  MlpRegs W;
+  mlp_load(P.mlp, W);
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  const uint64_t lane = threadIdx.x & 63;
+  for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); grp * 64 < n; grp += waves) {
+    const uint64_t i = grp * 64 + lane;
     const bool valid = i < n;
     double e[8];
     uint32_t meta = 0;
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
       meta = in.meta[i];
     }
     float y[5];
-    propose(P, valid, e, y);
+    propose(P, W, valid, e, y);
     if (valid) out[i] = proposal_byte(P, y, meta);
   }
 }
@@ -1449,10 +1457,16 @@ void set_family(RefineParams& P, const Plan& pl, int s) {
 
 // Proposal bytes of the items of `in` (n, or *P.in_count when set) into the
 // family's proposal region; the round launched next reads them through P.prop.
-void launch_propose(const Plan& pl, int s, RefineParams& P, const Worklist& in, uint64_t n, unsigned grid) {
+// n_expect: the round's item count (or its expected count for device-sized
+// rounds); the grid gives each wave about kProposeGroups 64-cell groups.
+constexpr uint64_t kProposeGroups = 8;
+void launch_propose(const Plan& pl, int s, RefineParams& P, const Worklist& in, uint64_t n, uint64_t n_expect) {
   P.prop = nullptr;
   if (!pl.learned) return;
   uint8_t* out = g.prop.as<uint8_t>() + (size_t)s * pl.cap;
+  const uint64_t per_block = 256 * kProposeGroups;
+  const uint64_t cap = (uint64_t)std::max(g.cu_count, 1) * 8;
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min((std::max<uint64_t>(n_expect, 1) + per_block - 1) / per_block, cap));
   hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, launch_stream(), P, in, n, out);
   P.prop = out;
 }
@@ -1552,7 +1566,7 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
         HIP_TRY(hipGetLastError());
       }
       P.prop = nullptr;
-      if (d > 0) launch_propose(pl, s, P, in, n, grid_for(n, 256));
+      if (d > 0) launch_propose(pl, s, P, in, n, n);
       HIP_TRY(hipEventRecord(a, g.stream));
       dispatch_family<LaunchRound>(pl.fams[s], d == 0, pl.ext, grid_for(n, 256), P, in, n);
       HIP_TRY(hipGetLastError());
@@ -1702,7 +1716,7 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
     P.out = region(g.wl[nxt], pl.cap, s);
     P.out_count = rot_counter(d + 1, s);
     P.zero_count = rot_counter(d + 2, s);
-    launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, grid_tuned(exp_of(d), pl));
+    launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, exp_of(d));
     dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d), pl), pl.block, P,
                                   region(g.wl[cur], pl.cap, s), (uint64_t)0);
   }

@@ -9,7 +9,7 @@
 // Parameters (FP32, this order): W1[8][64], b1[64], W2[64][64], b2[64],
 // W3[64][8], b3[8]; W[i][j] connects input i to unit j.
 //
-// Inference runs inside the evaluation kernels on one wave of 64 cells with
+// Inference runs on waves of 64 cells (k_propose, ahead of each round) with
 // v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulate).  Activations are
 // kept transposed (units x cells), so a layer's output registers are exactly
 // the next layer's B operand: a lane of group g = lane / 16 holds, for M-tile
@@ -65,9 +65,34 @@ __device__ __forceinline__ void mlp_features(const double* e, const double* box_
 
 __device__ __forceinline__ float mlp_shfl(float v, int src) { return __shfl(v, src, 64); }
 
+// One lane's operands of a family's network, held in registers so that a wave
+// can run many 64-cell groups on one load of the weights.
+struct MlpRegs {
+  bf16x8 a1[4];
+  f32x4 b1[4];
+  bf16x8 a2[4][2];
+  f32x4 b2[4];
+  bf16x8 a3[2];
+  f32x4 b3;
+};
+__device__ __forceinline__ void mlp_load(const MlpPacked* __restrict__ M, MlpRegs& W) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    W.a1[t] = M->a1[t][lane];
+    W.b1[t] = M->b1[t][lane];
+    W.a2[t][0] = M->a2[t][0][lane];
+    W.a2[t][1] = M->a2[t][1][lane];
+    W.b2[t] = M->b2[t][lane];
+  }
+  W.a3[0] = M->a3[0][lane];
+  W.a3[1] = M->a3[1][lane];
+  W.b3 = M->b3[lane];
+}
+
 // One wave: every lane passes its cell's features (zeros for idle lanes) and
 // receives outputs 0..4 of its own cell.  Must be called by all 64 lanes.
-__device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, const float* f, float* out5) {
+__device__ __forceinline__ void mlp_infer_wave(const MlpRegs& W, const float* f, float* out5) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, m = lane & 15;
   float res[4][5];
@@ -83,8 +108,8 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, 
     f32x4 h1[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      f32x4 c = M->b1[t][lane];
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M->a1[t][lane], x, c, 0, 0, 0);
+      f32x4 c = W.b1[t];
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a1[t], x, c, 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) c[r] = c[r] > 0.0f ? c[r] : 0.0f;
       h1[t] = c;
@@ -97,9 +122,9 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, 
     f32x4 h2[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      f32x4 c = M->b2[t][lane];
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M->a2[t][0][lane], b[0], c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M->a2[t][1][lane], b[1], c, 0, 0, 0);
+      f32x4 c = W.b2[t];
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a2[t][0], b[0], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a2[t][1], b[1], c, 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) c[r] = c[r] > 0.0f ? c[r] : 0.0f;
       h2[t] = c;
@@ -108,9 +133,9 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, 
     for (int s = 0; s < 2; ++s)
 #pragma unroll
       for (int i = 0; i < 8; ++i) b[s][i] = (__bf16)h2[2 * s + i / 4][i % 4];
-    f32x4 y = M->b3[lane];
-    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M->a3[0][lane], b[0], y, 0, 0, 0);
-    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(M->a3[1][lane], b[1], y, 0, 0, 0);
+    f32x4 y = W.b3;
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[0], b[0], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[1], b[1], y, 0, 0, 0);
     // outputs 0..3 of cell 16 j + m sit in lane m, output 4 in lane 16 + m
     const int src = lane & 15;
 #pragma unroll
@@ -126,6 +151,11 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, 
     v = mine == 3 ? res[3][k] : v;
     out5[k] = v;
   }
+}
+__device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, const float* f, float* out5) {
+  MlpRegs W;
+  mlp_load(M, W);
+  mlp_infer_wave(W, f, out5);
 }
 
 }  // namespace cpl
