@@ -10,6 +10,8 @@
 //    constants are rounded outward so every FP32 bound encloses the FP64 one).
 #pragma once
 
+#include <type_traits>
+
 #include "cpl_asin.hpp"
 
 namespace cpl {
@@ -76,6 +78,21 @@ template <class T>
 __device__ __forceinline__ bool asin_out_of_domain(IVT<T> x) {  // asin_iv's trap test
   return x.lb < T(-1) || x.ub > T(1);
 }
+
+// Single-endpoint quotients (FP64, W = 1 or 2).  For a >= 0 and 0 < o, both
+// non-empty, finite and o within [2^-500, 2^500), the interval a / o is
+// [RD(a.lb / o.ub), RU(a.ub / o.lb)] with lb <= ub, so its tests reduce to one
+// endpoint and exact comparisons: after tighten_ub(1) the quotient is empty iff
+// RD(a.lb / o.ub) > 1, i.e. a.lb >= succ(1) o.ub (the sign of one fma, exact in
+// this range); its upper bound exceeds 1 iff a.ub > o.lb.  Where every lane of
+// the wave meets these conditions the kernel computes only the quotient
+// endpoints its read endpoint needs (two directed divisions instead of four).
+__device__ __forceinline__ bool quot_fast_ok(IV a, IV o) {
+  return a.lb >= 0.0 && a.lb <= a.ub && a.ub < 0x1p+500 && o.lb >= 0x1p-500 && o.lb <= o.ub && o.ub < 0x1p+500;
+}
+__device__ __forceinline__ bool rd_quot_above_one(double a, double b) {  // RD(a / b) > 1
+  return __builtin_fma(-0x1.0000000000001p+0, b, a) >= 0.0;
+}
 template <class T, int W>
 __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> lc, IVT<T> sc, IVT<T> ab) {
   static_assert(W >= 1 && W <= 3, "endpoint selection");
@@ -99,24 +116,52 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
     if (sx.empty() || lx.empty()) {
       if (definitely(obt)) { trap = T_OBT_X; return Angle<T>{result, trap}; }
     } else {
-      I lquot = lx / lc;
-      lquot.tighten_ub(T(1));
-      I squot = sx / sc;
-      squot.tighten_ub(T(1));
-      if (lquot.empty() || squot.empty()) {
-        if (definitely(obt)) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
-      } else {
-        if (asin_out_of_domain(lquot) || asin_out_of_domain(squot)) {
-          trap = T_ASIN;
-          return Angle<T>{result, trap};
+      bool done = false;
+      if constexpr (std::is_same<T, double>::value && W != 3) {
+        if (all_lanes(definitely(obt) && quot_fast_ok(lx, lc) && quot_fast_ok(sx, sc))) {
+          // one endpoint of each quotient (quot_fast_ok); both lie in [0, 1], no domain trap
+          T lq, sq;
+          bool empty;
+          if (W == 1) {
+            lq = div_rd(lx.lb, lc.ub);  // lquot.lb
+            sq = div_ru(sx.ub, sc.lb);  // squot.ub, tightened to 1
+            if (T(1) < sq) sq = T(1);
+            empty = lq > T(1) || rd_quot_above_one(sx.lb, sc.ub);
+          } else {
+            lq = div_ru(lx.ub, lc.lb);  // lquot.ub, tightened to 1
+            if (T(1) < lq) lq = T(1);
+            sq = div_rd(sx.lb, sc.ub);  // squot.lb
+            empty = rd_quot_above_one(lx.lb, lc.ub) || sq > T(1);
+          }
+          if (empty) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
+          T rl = T(0), ru = T(0);
+          if (W == 1) rl = sub_rd(dasin_rd(lq), dasin_ru(sq));
+          else ru = sub_ru(dasin_ru(lq), dasin_rd(sq));
+          result = I(rl, ru);
+          result.tighten_lb(T(0));
+          done = true;
         }
-        // result = asin(lquot) - asin(squot), lower bound clamped at 0
-        const bool full = W == 3 || !definitely(obt);
-        T rl = T(0), ru = T(0);
-        if ((W & 1) || full) rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
-        if ((W & 2) || full) ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
-        result = I(rl, ru);
-        result.tighten_lb(T(0));
+      }
+      if (!done) {
+        I lquot = lx / lc;
+        lquot.tighten_ub(T(1));
+        I squot = sx / sc;
+        squot.tighten_ub(T(1));
+        if (lquot.empty() || squot.empty()) {
+          if (definitely(obt)) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
+        } else {
+          if (asin_out_of_domain(lquot) || asin_out_of_domain(squot)) {
+            trap = T_ASIN;
+            return Angle<T>{result, trap};
+          }
+          // result = asin(lquot) - asin(squot), lower bound clamped at 0
+          const bool full = W == 3 || !definitely(obt);
+          T rl = T(0), ru = T(0);
+          if ((W & 1) || full) rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
+          if ((W & 2) || full) ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
+          result = I(rl, ru);
+          result.tighten_lb(T(0));
+        }
       }
     }
   }
@@ -128,20 +173,41 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
     if (sx.empty()) {
       if (!possibly(obt)) { trap = T_ACU_X; return Angle<T>{result, trap}; }
     } else {
-      I lquot = lx / lc;
-      if (asin_out_of_domain(lquot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
-      I squot = sx / sc;
-      if (asin_out_of_domain(squot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
-      // r = asin(lquot) + asin(squot), the second's lower bound clamped at 0
-      I r(T(0), T(0));
-      if (W & 1) {
-        I beta(dasin_rd(squot.lb), T(0));
-        beta.tighten_lb(T(0));
-        r.lb = add_rd(dasin_rd(lquot.lb), beta.lb);
+      bool done = false;
+      if constexpr (std::is_same<T, double>::value && W != 3) {
+        if (all_lanes(quot_fast_ok(lx, lc) && quot_fast_ok(sx, sc))) {
+          // one endpoint of each quotient; upper bounds above 1 trap as asin_iv does
+          if (lx.ub > lc.lb) { trap = T_ASIN; return Angle<T>{result, trap}; }
+          if (sx.ub > sc.lb) { trap = T_ASIN; return Angle<T>{result, trap}; }
+          I r(T(0), T(0));
+          if (W == 1) {
+            I beta(dasin_rd(div_rd(sx.lb, sc.ub)), T(0));
+            beta.tighten_lb(T(0));
+            r.lb = add_rd(dasin_rd(div_rd(lx.lb, lc.ub)), beta.lb);
+          } else {
+            r.ub = add_ru(dasin_ru(div_ru(lx.ub, lc.lb)), dasin_ru(div_ru(sx.ub, sc.lb)));
+          }
+          if (result.empty()) result = r;
+          else result.do_join(r);
+          done = true;
+        }
       }
-      if (W & 2) r.ub = add_ru(dasin_ru(lquot.ub), dasin_ru(squot.ub));
-      if (result.empty()) result = r;
-      else result.do_join(r);
+      if (!done) {
+        I lquot = lx / lc;
+        if (asin_out_of_domain(lquot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
+        I squot = sx / sc;
+        if (asin_out_of_domain(squot)) { trap = T_ASIN; return Angle<T>{result, trap}; }
+        // r = asin(lquot) + asin(squot), the second's lower bound clamped at 0
+        I r(T(0), T(0));
+        if (W & 1) {
+          I beta(dasin_rd(squot.lb), T(0));
+          beta.tighten_lb(T(0));
+          r.lb = add_rd(dasin_rd(lquot.lb), beta.lb);
+        }
+        if (W & 2) r.ub = add_ru(dasin_ru(lquot.ub), dasin_ru(squot.ub));
+        if (result.empty()) result = r;
+        else result.do_join(r);
+      }
     }
   }
   return Angle<T>{result, trap};

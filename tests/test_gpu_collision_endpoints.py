@@ -7,7 +7,9 @@ Records are (lc, sc, ab) intervals built to cover every branch of the
 reference function: definitely obtuse, definitely acute, obtuseness
 undecided (both branches run; the first branch's result must then be computed
 in full because its emptiness decides between taking and joining the second),
-and the consistency traps (empty x or quotient intervals, arcsine domain)."""
+and the consistency traps (empty x or quotient intervals, arcsine domain).
+Records are also evaluated grouped by class, so that whole waves take the
+single-endpoint quotient paths that need every lane in the same case."""
 import numpy as np
 import pytest
 
@@ -42,9 +44,15 @@ def _obtuse_class(recs):
     return np.where(definitely, 2, np.where(possibly, 1, 0))
 
 
-def test_endpoint_forms_equal_full_computation_and_oracle(cpl, cplo):
+@pytest.mark.parametrize("grouped", [False, True])
+def test_endpoint_forms_equal_full_computation_and_oracle(cpl, cplo, grouped):
     recs = _records(17, 60_000)
     cls = _obtuse_class(recs)
+    if grouped:
+        # waves of one obtuseness class: the wave-uniform single-endpoint
+        # quotient paths (all lanes definitely obtuse / all acute) are taken
+        order = np.argsort(cls, kind="stable")
+        recs, cls = recs[order], cls[order]
     # every branch combination is represented
     for c in (0, 1, 2):
         assert np.count_nonzero(cls == c) > 2_000, c
