@@ -238,6 +238,8 @@ def main():
 
     # ---- roofline of the dominant kernel of the plain path (largest total device time) ----
     groups = kernel_groups(st_plain, plain_cfg)
+    # the published FP64 peak, checked by an fma-throughput probe on this device
+    peak_measured = cpl.fp64_peak(262144)  # ~30 ms of fma chains
     roof = None
     if groups:
         key = max(groups, key=lambda k: sum(s for _, s, _ in groups[k]))
@@ -264,6 +266,7 @@ def main():
             "avg_launch_ms": avg_sec * 1e3,
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
+            "peak_measured": peak_measured,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved is not None else None,
             "traffic": traffic,

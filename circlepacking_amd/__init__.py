@@ -34,6 +34,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.cpl_last_error.restype = ctypes.c_char_p
         L.cpl_debug_prim.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.c_size_t]
+        L.cpl_debug_fp64_peak.argtypes = [ctypes.c_int, _P_D]
         L.cpl_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8]
         L.cpl_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
         L.cpl_debug_geom.argtypes = [ctypes.c_int, ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int),
@@ -63,6 +64,13 @@ def _check(rc):
 
 def _dptr(a):
     return a.ctypes.data_as(_P_D)
+
+
+def fp64_peak(iters=65536):
+    """Measured FP64 FMA throughput of the device in TFLOP/s."""
+    t = ctypes.c_double()
+    _check(lib().cpl_debug_fp64_peak(int(iters), ctypes.byref(t)))
+    return t.value
 
 
 def prim(op, a, b=None):

@@ -1214,6 +1214,49 @@ int cpl_shutdown(void) {
   return CPL_OK;
 }
 
+// FP64 FMA throughput probe (checks the published FP64 peak the roofline uses):
+// eight independent fma chains per lane cover the pipeline latency; operands
+// come from arguments and the chains converge to 1, so nothing folds and the
+// final store never happens.
+__global__ __launch_bounds__(256) void k_fp64_peak(int iters, double b, double c, double* out) {
+  double a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = (double)(threadIdx.x + k) * 1e-3;
+#pragma unroll 4
+  for (int i = 0; i < iters; ++i)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = __builtin_fma(a[k], b, c);
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k];
+  if (s == 12345.0) out[0] = s;
+}
+
+int cpl_debug_fp64_peak(int iters, double* tflops) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (iters <= 0 || !tflops) return fail(CPL_ERR_ARG, "cpl_debug_fp64_peak: bad argument");
+  if ((rc = g.scratch_out.ensure(sizeof(double)))) return rc;
+  const unsigned blocks = (unsigned)std::max(g.cu_count, 1) * 8;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_fp64_peak, dim3(blocks), dim3(256), 0, g.stream, std::min(iters, 1024), 0.999, 1e-3,
+                     g.scratch_out.as<double>());  // warm-up
+  HIP_TRY(hipEventRecord(e0, g.stream));
+  hipLaunchKernelGGL(k_fp64_peak, dim3(blocks), dim3(256), 0, g.stream, iters, 0.999, 1e-3, g.scratch_out.as<double>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, g.stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double flops = (double)blocks * 256.0 * (double)iters * 8.0 * 2.0;
+  *tflops = ms > 0.f ? flops / (ms * 1e-3) / 1e12 : 0.0;
+  return CPL_OK;
+}
+
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;

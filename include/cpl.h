@@ -204,6 +204,11 @@ int cpl_autotune(cpl_refine_cfg* cfg, int reps, cpl_tune_point* points, int* n_p
  * 11 dasin_rd 12 dasin_ru 13 lower_bound_radius_single_placement(a)
  * 14 div_rn 15 sqrt_rn. */
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n);
+
+/* No counterpart in the reference: FP64 FMA throughput of the device (eight
+ * independent fma chains per lane, `iters` fmas per chain), in TFLOP/s (fma =
+ * 2 flops).  A check of the published FP64 peak that roofline figures use. */
+int cpl_debug_fp64_peak(int iters, double* tflops);
 /* FP32 prefilter primitives through the same entry point (operands and
  * results are float values carried in doubles): 16 add_rd 17 add_ru 18 sub_rd
  * 19 sub_ru 20 mul_rd 21 mul_ru 22 div_rd 23 div_ru 24 asin_rn 25 dasin_rd

@@ -64,3 +64,11 @@ def test_asin_endpoints(cpl):
 def test_single_placement_bound(cpl, cplo):
     lam = np.concatenate([np.linspace(0.5, 0.9900000000000001, 2049), [0.0, 0.25, 0.999]])
     assert_bit_exact(cpl.prim("lbrsp", lam), cplo.prim("lbrsp", lam), lam, None, "lbrsp")
+
+
+def test_fp64_peak_probe_is_plausible(cpl):
+    # eight fma chains per lane on every CU: the published FP64 peak of an
+    # MI355X is 78.6 TFLOP/s; the probe must land near it (clock and power
+    # management move it by a few percent)
+    t = cpl.fp64_peak(16384)
+    assert 40.0 < t < 90.0, t
