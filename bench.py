@@ -270,6 +270,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved is not None else None,
             "traffic": traffic,
+            "traffic_gbps": (traffic / avg_sec / 1e9) if traffic is not None else None,
             "worklist_items_per_launch": items,
         }
 
