@@ -70,3 +70,22 @@ def test_autotune_picks_a_listed_shape(cpl):
     assert len(table) == 20
     best = min(table, key=lambda t: t[2])
     assert (cfg.block, cfg.cells_per_thread) == best[:2]
+
+
+def test_graph_runs_report_only_their_families(cpl, cplo):
+    # each family chain of the graph form clears its own statistics records, so
+    # records that a run with other families left behind must not reach a run
+    # without them (with and without per-root flags, i.e. with and without the
+    # shared prologue)
+    four = abi.with_options(abi.cfg_4param(grid=(16, 16, 16, 16)), G | H)
+    cases = [(abi.cfg_4param(grid=(16, 16, 16, 16), families=(abi.MIDDLE, abi.END_3D)), G | H),
+             (abi.cfg_small_2param(), G)]
+    for cfg, opts in cases:
+        cpl.refine_run(four)
+        plain = cpl.refine_run(cfg)
+        same(plain, cplo.refine_run(cfg))
+        for flags in (False, True):
+            s, f = cpl.refine_run(abi.with_options(cfg, opts), want_flags=flags)
+            assert s.as_dict() == plain[0].as_dict()
+            if flags:
+                assert np.array_equal(f, plain[1])
