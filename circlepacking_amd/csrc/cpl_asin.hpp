@@ -32,18 +32,20 @@ __device__ __forceinline__ double asin_poly_eval(double z) {
   return p;
 }
 
+// Both ranges evaluate the one polynomial (in x^2 below 1/2, in t = (1 - |x|)/2
+// above), so a wave whose lanes fall in both ranges evaluates it once.
 __device__ __forceinline__ double asin_rn(double x) {
   double ax = __builtin_fabs(x);
+  const bool small = ax <= 0.5;
+  double t = 0.5 - 0.5 * ax;  // exact for ax in [1/2, 1]
+  double z = small ? ax * ax : t;
+  double p = asin_poly_eval(z);
   double r;
-  if (ax <= 0.5) {
-    double z = ax * ax;
-    double p = asin_poly_eval(z);
+  if (small) {
     r = __builtin_fma(ax * z, p, ax);
   } else {
-    double t = 0.5 - 0.5 * ax;  // exact for ax in [1/2, 1]
     double s = __builtin_sqrt(t);
     double c = t > 0.0 ? __builtin_fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
-    double p = asin_poly_eval(t);
     double w = __builtin_fma(s * t, p, c);
     r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));
   }
@@ -111,14 +113,16 @@ __device__ __forceinline__ float asin32_poly(float z) {
 
 __device__ __forceinline__ float asin32_rn(float x) {
   float ax = __builtin_fabsf(x);
+  const bool small = ax <= 0.5f;
+  float t = 0.5f - 0.5f * ax;
+  float z = small ? ax * ax : t;
+  float p = asin32_poly(z);
   float r;
-  if (ax <= 0.5f) {
-    float z = ax * ax;
-    r = __builtin_fmaf(ax * z, asin32_poly(z), ax);
+  if (small) {
+    r = __builtin_fmaf(ax * z, p, ax);
   } else {
-    float t = 0.5f - 0.5f * ax;
     float s = __builtin_sqrtf(t);
-    float w = __builtin_fmaf(s * t, asin32_poly(t), 0.0f);
+    float w = __builtin_fmaf(s * t, p, 0.0f);
     r = 0x1.921fb6p+0f - ((s + s) + (w + w + 0x1.777a5cp-25f));
   }
   return __builtin_copysignf(r, x);

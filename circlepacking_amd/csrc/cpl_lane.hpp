@@ -59,13 +59,7 @@ struct Angle {
   int trap;
 };
 
-// collision_angles.cu:29-31
-template <class T>
-__device__ __forceinline__ UB is_obtuse(IVT<T> hyp, IVT<T> s1, IVT<T> s2) {
-  return hyp.square() > s1.square() + s2.square();
-}
-
-// collision_angles.cu:37-117
+// collision_angles.cu:37-117 (with the obtuseness test of :29-31)
 //
 // W names the endpoints of the result the caller reads (bit 0: lb, bit 1: ub;
 // every caller on the path reads one).  An arcsine that reaches only an unread
@@ -101,9 +95,11 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
   I result(T(0), T(-1));
-  UB obt = is_obtuse(lc, sc, ab);
   I lc_sq = lc.square();
   I sc_sq = sc.square();
+  // is_obtuse (collision_angles.cu:29-31) on the squares computed once: the
+  // directed-rounding statements are not merged by the compiler
+  UB obt = lc_sq > sc_sq + ab.square();
   I add_term = (lc_sq - sc_sq) / ab;
   add_term.tighten_lb(T(0));
 

@@ -512,10 +512,8 @@ static inline IV get_subinterval(IV iv, int i, int n) {
 // ---------------------------------------------------------------------------
 // Lane geometry (lane_packing/collision_angles.cu, lower_bound_radius_single_placement.cu)
 // ---------------------------------------------------------------------------
-// collision_angles.cu:29-31
-static inline UB is_obtuse(IV hypot, IV s1, IV s2) { return hypot.square() > s1.square() + s2.square(); }
-
-// collision_angles.cu:37-117
+// collision_angles.cu:37-117; the obtuseness test is collision_angles.cu:29-31
+// (is_obtuse) on the squares the function computes anyway.
 //
 // `reads` (bit 0: lb, bit 1: ub) names the endpoints of the result the caller
 // uses; it changes no value, only the flop count of the counting build: an
@@ -527,9 +525,9 @@ static inline IV compute_collision_angle(IV lc, IV sc, IV ab, int reads = 3) {
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
   IV result(0.0, -1.0);
-  UB obt = is_obtuse(lc, sc, ab);
   IV lc_sq = lc.square();
   IV sc_sq = sc.square();
+  UB obt = lc_sq > sc_sq + ab.square();  // is_obtuse(lc, sc, ab), squares computed once
   IV add_term = (lc_sq - sc_sq) / ab;
   add_term.tighten_lb(0.0);
 
