@@ -978,53 +978,64 @@ __device__ __forceinline__ unsigned long long ordered_of(double v) {
   return b ^ mask;
 }
 
+// The grid in the reference's cell order (i3 fastest, then i2, i1, la), walked
+// as rows: a block takes one (la, i1) row at a time (grid stride over rows) and
+// its threads step through the row's n2 * n3 cells 256 apart with a mixed-radix
+// counter, so a cell costs no 64-bit division and the row's r1 subinterval and
+// r2 range are computed once per row.  The lanes of a wave hold consecutive
+// cells, as with one flat index per thread.
 template <int F>
 __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
-  const uint64_t n3 = (uint64_t)P.n[3], n2 = (uint64_t)P.n[2], n1 = (uint64_t)P.n[1];
-  const uint64_t total = (uint64_t)P.n[0] * n1 * n2 * n3;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint32_t n1 = (uint32_t)P.n[1], n2 = (uint32_t)P.n[2], n3 = (uint32_t)P.n[3];
+  const uint64_t rows = (uint64_t)P.n[0] * n1;
+  // start cell and step of this thread within a row, as (i2, i3) digits
+  const uint32_t s2 = threadIdx.x / n3, s3 = threadIdx.x % n3;
+  const uint32_t d2 = blockDim.x / n3, d3 = blockDim.x % n3;
   unsigned long long best = ~0ull, ev = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
-    uint64_t i = base + threadIdx.x;
-    if (i >= total) continue;
-    uint64_t t = i;
-    int i3 = (int)(t % n3); t /= n3;
-    int i2 = (int)(t % n2); t /= n2;
-    int i1 = (int)(t % n1); t /= n1;
-    int ia = (int)t;
+  for (uint64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int ia = (int)(row / n1), i1 = (int)(row % n1);
     const double* rr = P.r1_ranges + 4 * ia;
-    IV la(rr[0], rr[1]);
-    IV r1_ = get_subinterval(IV(rr[2], rr[3]), i1, P.n[1]);
-    IV r2_ = get_subinterval(r2_range<F>(la, r1_), i2, P.n[2]);
-    IV r3(0.0, 0.0);
-    if constexpr (is_3d(F)) {
-      IV range = r3_range<F>(la, r2_);
-      if (F == F_END_3D && range.empty()) continue;  // end_configuration.cu:222-224
-      r3 = get_subinterval(range, i3, P.n[3]);
-    }
-    uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3);
-    ++ev;
-    int v = vb & 7;
-    if (v == V_CRITICAL) {
-      unsigned long long o = ordered_of(la.lb);
-      best = o < best ? o : best;
-      if (P.output_criticals) {
-        uint32_t slot = atomicAdd(P.crit_count, 1u);
-        if (slot < P.crit_cap) {
-          double* rec = P.crit_out + 9 * (size_t)slot;
-          IV r2 = r2_;
-          if (is_3d(F)) r2.tighten_lb(r3.lb);
-          IV r1 = r1_;
-          r1.tighten_lb(r2.lb);
-          rec[0] = (double)(vb >> 4);
-          rec[1] = la.lb; rec[2] = la.ub; rec[3] = r1.lb; rec[4] = r1.ub;
-          rec[5] = r2.lb; rec[6] = r2.ub; rec[7] = r3.lb; rec[8] = r3.ub;
-        }
+    const IV la(rr[0], rr[1]);
+    const IV r1_ = get_subinterval(IV(rr[2], rr[3]), i1, P.n[1]);
+    const IV r2rng = r2_range<F>(la, r1_);
+    for (uint32_t i2 = s2, i3 = s3; i2 < n2;) {
+      const IV r2_ = get_subinterval(r2rng, (int)i2, P.n[2]);
+      IV r3(0.0, 0.0);
+      bool skip = false;
+      if constexpr (is_3d(F)) {
+        IV range = r3_range<F>(la, r2_);
+        if (F == F_END_3D && range.empty()) skip = true;  // end_configuration.cu:222-224
+        else r3 = get_subinterval(range, (int)i3, P.n[3]);
       }
-    } else if (v == V_NOFIT) {
-      atomicOr(P.status + 1, 1u);
-    } else if (v == V_TRAP) {
-      atomicCAS(P.status, 0u, (uint32_t)(vb >> 4));
+      // next cell of this thread (i3 + d3 < 2 n3, one carry at most)
+      i3 += d3;
+      i2 += d2;
+      if (i3 >= n3) { i3 -= n3; ++i2; }
+      if (skip) continue;
+      uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3);
+      ++ev;
+      int v = vb & 7;
+      if (v == V_CRITICAL) {
+        unsigned long long o = ordered_of(la.lb);
+        best = o < best ? o : best;
+        if (P.output_criticals) {
+          uint32_t slot = atomicAdd(P.crit_count, 1u);
+          if (slot < P.crit_cap) {
+            double* rec = P.crit_out + 9 * (size_t)slot;
+            IV r2 = r2_;
+            if (is_3d(F)) r2.tighten_lb(r3.lb);
+            IV r1 = r1_;
+            r1.tighten_lb(r2.lb);
+            rec[0] = (double)(vb >> 4);
+            rec[1] = la.lb; rec[2] = la.ub; rec[3] = r1.lb; rec[4] = r1.ub;
+            rec[5] = r2.lb; rec[6] = r2.ub; rec[7] = r3.lb; rec[8] = r3.ub;
+          }
+        }
+      } else if (v == V_NOFIT) {
+        atomicOr(P.status + 1, 1u);
+      } else if (v == V_TRAP) {
+        atomicCAS(P.status, 0u, (uint32_t)(vb >> 4));
+      }
     }
   }
   // wave min / sum, one atomic each
@@ -2140,8 +2151,9 @@ int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, in
   hipLaunchKernelGGL(k_mode_r_ranges, dim3((unsigned)((n[0] + 255) / 256)), dim3(256), 0, g.stream, P.outer, n[0],
                      g.mr_ranges.as<double>());
   HIP_TRY(hipGetLastError());
-  const uint64_t total = (uint64_t)n[0] * n[1] * n[2] * n[3];
-  dispatch_family<LaunchModeR>(family, grid_for(total, 256), P);
+  // one block per (la, r1) row, grid stride over rows
+  const uint64_t rows = (uint64_t)n[0] * n[1];
+  dispatch_family<LaunchModeR>(family, (unsigned)std::min<uint64_t>(rows, (uint64_t)std::max(g.cu_count, 1) * 16), P);
   HIP_TRY(hipGetLastError());
   unsigned long long back[4];
   std::vector<unsigned long long> evs(64 * 16);
