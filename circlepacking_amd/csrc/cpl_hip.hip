@@ -981,9 +981,9 @@ __device__ __forceinline__ unsigned long long ordered_of(double v) {
 // The grid in the reference's cell order (i3 fastest, then i2, i1, la), walked
 // as rows: a block takes one (la, i1) row at a time (grid stride over rows) and
 // its threads step through the row's n2 * n3 cells 256 apart with a mixed-radix
-// counter, so a cell costs no 64-bit division and the row's r1 subinterval and
-// r2 range are computed once per row.  The lanes of a wave hold consecutive
-// cells, as with one flat index per thread.
+// counter, so a cell costs no 64-bit division and the row's r1 subinterval,
+// r2 range and semicircle angle of r1 are computed once per row.  The lanes of
+// a wave hold consecutive cells, as with one flat index per thread.
 template <int F>
 __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
   const uint32_t n1 = (uint32_t)P.n[1], n2 = (uint32_t)P.n[2], n3 = (uint32_t)P.n[3];
@@ -998,6 +998,11 @@ __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
     const IV la(rr[0], rr[1]);
     const IV r1_ = get_subinterval(IV(rr[2], rr[3]), i1, P.n[1]);
     const IV r2rng = r2_range<F>(la, r1_);
+    // the row's semicircle angle of r1 for the criteria that use it on most cells
+    // (the start family's criteria reach it rarely; holding it there costs more)
+    constexpr bool kA1 = F == F_START_END_2D || F == F_START_END_3D || F == F_END_2D;
+    IV a1r(0.0, 0.0);
+    if constexpr (kA1) a1r = semicircle_angle_outer(r1_);
     for (uint32_t i2 = s2, i3 = s3; i2 < n2;) {
       const IV r2_ = get_subinterval(r2rng, (int)i2, P.n[2]);
       IV r3(0.0, 0.0);
@@ -1012,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
       i2 += d2;
       if (i3 >= n3) { i3 -= n3; ++i2; }
       if (skip) continue;
-      uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3);
+      uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3, true, nullptr, nullptr, kA1 ? &a1r : nullptr);
       ++ev;
       int v = vb & 7;
       if (v == V_CRITICAL) {

@@ -347,6 +347,15 @@ __device__ __forceinline__ IVT<T> semicircle_angle_inner(IVT<T> la, IVT<T> r) {
   return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))), dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
 }
 
+// semicircle_angle_outer(r1), or `row` when it holds that angle for an r1 with
+// the same bits (the reference-grid kernel computes it once per (la, r1) row;
+// tightening leaves r1 unchanged for nearly every cell of the row).
+template <class T>
+__device__ __forceinline__ IVT<T> semicircle_outer_r1(IVT<T> r1, const IVT<T>* row) {
+  if (row) return *row;
+  return semicircle_angle_outer(r1);
+}
+
 // lower_bound_radius_single_placement.cu:26-61
 __device__ __attribute__((noinline)) double lower_bound_radius_single_placement(double lb_lambda) {
   const double ub_rhs_factor = div_ru(critical_ratio, lb_pi);
@@ -401,28 +410,29 @@ __device__ __forceinline__ T crit_of(T x) { return mul_ru(LaneConst<T>::crit, x)
 
 // start_configuration.cu:93-101
 template <class T>
-__device__ __forceinline__ bool handle_case_1(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> a12) {
+__device__ __forceinline__ bool handle_case_1(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> a12, const IVT<T>* a1_row) {
   T ub_ratio = ub_ratio_of(la);
   T lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2);
-  IVT<T> a = semicircle_angle_outer(r1) + a12;
+  IVT<T> a = semicircle_outer_r1(r1, a1_row) + a12;
   T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), a.ub));
   T lb_disk_area = add_rd(lb_area1, mul_rd(T(0.5), lb_area2));
   return lb_disk_area >= crit_of(ub_area_used);
 }
 // start_configuration.cu:103-112
 template <class T>
-__device__ __forceinline__ bool handle_case_2(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, IVT<T> a13) {
+__device__ __forceinline__ bool handle_case_2(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, IVT<T> a13,
+                                              const IVT<T>* a1_row) {
   T ub_ratio = ub_ratio_of(la);
   T lb_area1 = lb_disk(r1), lb_area2 = lb_disk(r2), lb_area3 = lb_disk(r3);
-  IVT<T> a = semicircle_angle_outer(r1) + a13;
+  IVT<T> a = semicircle_outer_r1(r1, a1_row) + a13;
   T ub_area_used = mul_ru(ub_ratio, mul_ru(T(0.5), a.ub));
   T lb_disk_area = add_rd(add_rd(lb_area1, lb_area2), mul_rd(T(0.5), lb_area3));
   return lb_disk_area >= crit_of(ub_area_used);
 }
 // start_configuration.cu:169-188
 template <class T>
-__device__ __forceinline__ bool handle_2d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap) {
-  IVT<T> a1 = semicircle_angle_outer(r1);
+__device__ __forceinline__ bool handle_2d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap, const IVT<T>* a1_row) {
+  IVT<T> a1 = semicircle_outer_r1(r1, a1_row);
   IVT<T> a2 = semicircle_angle_inner(la, r2);
   IVT<T> a12 = collision_angle_outer_inner(la, r1, r2, trap);
   if (trap) return false;
@@ -438,9 +448,10 @@ __device__ __forceinline__ bool handle_2d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, 
 }
 // start_configuration.cu:227-265 (outer_outer(r1, r2) at :232 kept as in the reference)
 template <class T>
-__device__ __forceinline__ bool handle_3d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap) {
+__device__ __forceinline__ bool handle_3d_case(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, int& trap,
+                                               const IVT<T>* a1_row) {
   using I = IVT<T>;
-  I a1 = semicircle_angle_outer(r1);
+  I a1 = semicircle_outer_r1(r1, a1_row);
   I a2 = semicircle_angle_inner(la, r2);
   I a3 = semicircle_angle_outer(r3);
   I a12 = collision_angle_outer_inner(la, r1, r2, trap);
@@ -521,13 +532,14 @@ __device__ __forceinline__ bool handle_inner_outer_2d(IVT<T> la, IVT<T> r1, IVT<
 }
 // end_configuration.cu:87-101
 template <class T>
-__device__ __forceinline__ bool handle_outer_inner_2d(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap) {
+__device__ __forceinline__ bool handle_outer_inner_2d(IVT<T> la, IVT<T> r1, IVT<T> r2, int& trap,
+                                                      const IVT<T>* a1_row) {
   IVT<T> a = collision_angle_outer_inner(la, r1, r2, trap) + semicircle_angle_inner(la, r2);
   if (trap) return false;
   T ub_area_used = mul_ru(ub_ratio_of(la), mul_ru(T(0.5), a.ub));
   IVT<T> rho = T(1) - (la + T(2) * r2).square();
   rho.tighten_lb(T(0));
-  T lb_alpha_reusable = (a - semicircle_angle_outer(r1)).lb;
+  T lb_alpha_reusable = (a - semicircle_outer_r1(r1, a1_row)).lb;
   T lb_area_reusable = mul_rd(mul_rd(T(0.5), lb_alpha_reusable), rho.lb);
   T lb_disk_area = add_rd(lb_disk(r2), mul_rd(T(0.5), lb_disk(r1)));
   return lb_disk_area >= crit_of(add_ru(ub_area_used, -lb_area_reusable));
@@ -581,8 +593,9 @@ __device__ __forceinline__ bool handle_r1_outer_3d(IVT<T> la, IVT<T> r1, IVT<T> 
 // the 1-based failing criterion, or 16 + trap code.  Inline: out of line, the
 // call would save and restore ~40 callee-saved VGPRs per lane through scratch
 // (measured: HBM traffic of the round kernel x1.8 for no gain in time).
+// a1_row: semicircle_angle_outer(r1) when the caller has it (null otherwise).
 template <int F, class T>
-__device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3) {
+__device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, const IVT<T>* a1_row = nullptr) {
   int trap = 0;
   if constexpr (F == F_START) {  // start_configuration.cu:142-162
     IVT<T> a12 = collision_angle_outer_inner(la, r1, r2, trap);
@@ -592,15 +605,15 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
     IVT<T> a23 = collision_angle_inner_outer(la, r2, r3, trap);
     if (trap) return 16 + trap;
     UB case1 = (a12 + a23 > a13);
-    if (possibly(case1) && !handle_case_1(la, r1, r2, a12)) return 1;
-    if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13)) return 2;
+    if (possibly(case1) && !handle_case_1(la, r1, r2, a12, a1_row)) return 1;
+    if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13, a1_row)) return 2;
     return 0;
   } else if constexpr (F == F_START_END_2D) {
-    bool ok = handle_2d_case(la, r1, r2, trap);
+    bool ok = handle_2d_case(la, r1, r2, trap, a1_row);
     if (trap) return 16 + trap;
     return ok ? 0 : 1;
   } else if constexpr (F == F_START_END_3D) {
-    bool ok = handle_3d_case(la, r1, r2, r3, trap);
+    bool ok = handle_3d_case(la, r1, r2, r3, trap, a1_row);
     if (trap) return 16 + trap;
     return ok ? 0 : 1;
   } else if constexpr (F == F_MIDDLE) {
@@ -614,7 +627,7 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
     bool ok = handle_inner_outer_2d(la, r1, r2, trap);
     if (trap) return 16 + trap;
     if (!ok) return 1;
-    ok = handle_outer_inner_2d(la, r1, r2, trap);
+    ok = handle_outer_inner_2d(la, r1, r2, trap, a1_row);
     if (trap) return 16 + trap;
     return ok ? 0 : 2;
   } else {
@@ -634,9 +647,12 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
 // FP32 outward enclosure of the tightened cell; a cell FP32 proves is PROVEN,
 // every other cell is decided by the FP64 evaluation.  `*fp32_hit` is set when
 // FP32 settled the cell, `*fp32_tried` when the FP32 criteria ran.
+// a1_row (reference-grid kernel): semicircle_angle_outer(r1_), used where
+// tightening leaves r1 = r1_ bit for bit.
 template <int F, bool PREFILTER = false>
 __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3, bool pre = true,
-                                                    bool* fp32_hit = nullptr, bool* fp32_tried = nullptr) {
+                                                    bool* fp32_hit = nullptr, bool* fp32_tried = nullptr,
+                                                    const IV* a1_row = nullptr) {
   IV r2 = r2_;
   if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
   IV r1 = r1_;
@@ -659,7 +675,8 @@ __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3
       }
     }
   }
-  int c = criteria<F, double>(la, r1, r2, r3);
+  const IV* a1 = (a1_row && dbits(r1.lb) == dbits(r1_.lb) && dbits(r1.ub) == dbits(r1_.ub)) ? a1_row : nullptr;
+  int c = criteria<F, double>(la, r1, r2, r3, a1);
   if (c >= 16) return vbyte(V_TRAP, nofit, c - 16);
   if (c) return vbyte(V_CRITICAL, nofit, c);
   return vbyte(V_PROVEN, nofit, 0);
