@@ -985,7 +985,7 @@ __device__ __forceinline__ unsigned long long ordered_of(double v) {
 // r2 range and semicircle angle of r1 are computed once per row.  The lanes of
 // a wave hold consecutive cells, as with one flat index per thread.
 template <int F>
-__global__ __launch_bounds__(256) void k_mode_r(ModeRParams P) {
+__global__ __launch_bounds__(256, 3) void k_mode_r(ModeRParams P) {
   const uint32_t n1 = (uint32_t)P.n[1], n2 = (uint32_t)P.n[2], n3 = (uint32_t)P.n[3];
   const uint64_t rows = (uint64_t)P.n[0] * n1;
   // start cell and step of this thread within a row, as (i2, i3) digits
