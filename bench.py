@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--forms", default="plain,graph,persistent,fp32",
                     help="forms to time besides plain (graph, persistent, fp32); the learned forms follow --no-learned")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-reference-grid", action="store_true",
+                    help="skip the one-launch-per-family timing on the reference grids")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 256 MiB device copy three times and one plain run, then exit (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -274,6 +276,20 @@ def main():
             "worklist_items_per_launch": items,
         }
 
+    # ---- the reference's own kernels: one launch per family over [lambda_min, lambda_max] on its
+    # full grids (cpl_first_problematic, the call its prover makes per lambda interval) ----
+    reference_grid = None
+    if not args.no_reference_grid:
+        reference_grid = {}
+        lam = (abi.LAMBDA_MIN, abi.LAMBDA_MAX)
+        cpl.first_problematic(0, lam)  # warm-up
+        for f in range(6):
+            t0 = time.perf_counter()
+            first, ev, _ = cpl.first_problematic(f, lam)
+            dt = time.perf_counter() - t0
+            reference_grid[abi.FAMILY_NAMES[f]] = {"ms": dt * 1e3, "cells": ev, "cells_per_s": ev / dt,
+                                                   "first_problematic_la": first}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -310,6 +326,7 @@ def main():
         "unresolved": st_plain.unresolved,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_grid_launch": reference_grid,
     }
     print(json.dumps(out), flush=True)
 
