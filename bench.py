@@ -248,7 +248,7 @@ def main():
         launches = groups[key]
         avg_sec = sum(s for _, s, _ in launches) / len(launches)
         f, is_root = key
-        kname = f"k_round<{f}, {'true' if is_root else 'false'}, 256, false>"
+        kname = f"k_round<{f}, {'true' if is_root else 'false'}, 256, false, 3>"  # the plain form's instance
         achieved = None
         if flops_per_eval is not None:
             flops_per_launch = sum(float(np.dot(c, flops_per_eval[f])) for _, _, c in launches) / len(launches)
