@@ -987,7 +987,7 @@ __device__ __forceinline__ unsigned long long ordered_of(double v) {
 // counter, so a cell costs no 64-bit division and the row's r1 subinterval,
 // r2 range and semicircle angle of r1 are computed once per row.  The lanes of
 // a wave hold consecutive cells, as with one flat index per thread.  Four waves
-// per SIMD (<= 128 VGPRs, a few spills outside the criteria' hot path) beat
+// per SIMD (<= 128 VGPRs, a few spills outside the criteria's hot path) beat
 // three by 6-13 % on every family; five lose again.
 template <int F>
 __global__ __launch_bounds__(256, 4) void k_mode_r(ModeRParams P) {
