@@ -105,7 +105,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md and not built", file=sys.stderr)
+              "sharding the proof over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     import circlepacking_amd as cpl
