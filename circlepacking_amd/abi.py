@@ -45,6 +45,8 @@ PRIM_TRIG_OPS = {
     "sin_rn": 29, "cos_rn": 30, "acos_rn": 31, "dsin_rd": 32, "dsin_ru": 33, "dcos_rd": 34, "dcos_ru": 35,
     "dacos_rd": 36, "dacos_ru": 37,
 }
+# Atomic_ordered_double's transform and its inverse (the u64 image carried in a double's bits)
+PRIM_ORDERED_OPS = {"ordered": 38, "ordered_inv": 39}
 IV_OPS = {
     "add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "abs": 6, "reciprocal": 7,
     "div_scalar": 8, "scalar_div": 9, "sin": 10, "cos": 11, "asin": 12, "acos": 13, "mul_scalar": 14, "neg": 15,
@@ -53,7 +55,7 @@ IV_BINARY = {"add", "sub", "mul", "div", "div_scalar", "scalar_div", "mul_scalar
 
 
 def prim_code(op):
-    for table in (PRIM_OPS, PRIM32_OPS, PRIM_TRIG_OPS):
+    for table in (PRIM_OPS, PRIM32_OPS, PRIM_TRIG_OPS, PRIM_ORDERED_OPS):
         if op in table:
             return table[op]
     raise KeyError(op)

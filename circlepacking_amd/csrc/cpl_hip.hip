@@ -80,6 +80,8 @@ __global__ void k_debug_prim(int op, const double* a, const double* b, double* o
     case 35: r = dcos_ru(x); break;
     case 36: r = dacos_rd(x); break;
     case 37: r = dacos_ru(x); break;
+    case 38: r = dfrom(ordered_of(x)); break;         // the u64 image carried in the bits
+    case 39: r = double_of_ordered(dbits(x)); break;  // inverse of the image in x's bits
     default: r = __builtin_nan("");
   }
   out[i] = r;
@@ -974,13 +976,6 @@ __global__ void k_mode_r_ranges(IV outer, int nla, double* out) {
   out[4 * i + 3] = r.ub;
 }
 
-// Atomic_ordered_double::transform (atomic_float.hpp:56-60)
-__device__ __forceinline__ unsigned long long ordered_of(double v) {
-  uint64_t b = dbits(v);
-  uint64_t mask = (0 - (b >> 63)) | (1ull << 63);
-  return b ^ mask;
-}
-
 // The grid in the reference's cell order (i3 fastest, then i2, i1, la), walked
 // as rows: a block takes one (la, i1) row at a time (grid stride over rows) and
 // its threads step through the row's n2 * n3 cells 256 apart with a mixed-radix
@@ -1287,7 +1282,7 @@ int cpl_debug_fp64_peak(int iters, double* tflops) {
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (op < 0 || op > 37 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
+  if (op < 0 || op > 39 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
   if (n == 0) return CPL_OK;
   size_t bytes = n * sizeof(double);
   if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)))

@@ -101,6 +101,18 @@ __device__ __forceinline__ double fma_sc(double a, double b, double c) {
 __device__ __forceinline__ uint64_t dbits(double d) { return __double_as_longlong(d); }
 __device__ __forceinline__ double dfrom(uint64_t b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ int dexp_field(double d) { return (int)((dbits(d) >> 52) & 0x7FF); }
+// Atomic_ordered_double's order-preserving map (atomic_float.hpp:56-66): an
+// unsigned comparison of the images orders the doubles, so atomicMin on the
+// u64 is a minimum over doubles.
+__device__ __forceinline__ unsigned long long ordered_of(double v) {
+  uint64_t b = dbits(v);
+  uint64_t mask = (0 - (b >> 63)) | (1ull << 63);
+  return b ^ mask;
+}
+__device__ __forceinline__ double double_of_ordered(unsigned long long v) {
+  uint64_t mask = ((v >> 63) - 1) | (1ull << 63);
+  return dfrom(v ^ mask);
+}
 // 2^k for -1022 <= k <= 1023 (normal powers of two only)
 __device__ __forceinline__ double pow2(int k) { return dfrom((uint64_t)(k + 1023) << 52); }
 

@@ -215,7 +215,9 @@ int cpl_debug_fp64_peak(int iters, double* tflops);
  * 26 dasin_ru; 27 / 28: a double rounded down / up to float.  Round-to-nearest
  * and widened functions of the interval surface (argument in [0, 8] for sin and
  * cos): 29 sin_rn 30 cos_rn 31 acos_rn 32 dsin_rd 33 dsin_ru 34 dcos_rd
- * 35 dcos_ru 36 dacos_rd 37 dacos_ru. */
+ * 35 dcos_ru 36 dacos_rd 37 dacos_ru.  The ordered minimum's bit map
+ * (atomic_float.hpp:56-66): 38 the u64 image of a carried in out's bits,
+ * 39 the double whose image is the u64 carried in a's bits. */
 
 /* Elementwise interval operations of the algcuda::Interval<double> surface
  * (interval.cuh) on n intervals; b is the second operand (binary ops; for the

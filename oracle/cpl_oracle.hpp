@@ -17,14 +17,27 @@
 //
 // Build flags: -O2 -mfma -ffp-contract=off -fno-fast-math (see Makefile).
 //
-// Functions whose parity is pinned by tests (tests/test_oracle_*.py):
-//   primitives   exact-rational RD/RU golden vectors (tests/golden/prims_*.npz)
-//   asin_rn      mpmath ulp error and containment of the widened enclosure
+// How each part is pinned (tests/):
+//   primitives   exact-rational RD/RU golden vectors (golden/prims.npz,
+//                test_oracle_prims.py)
+//   asin_rn      mpmath ulp error and enclosure of the widened result
+//                (test_oracle_asin.py)
 //   lbrsp        closed-form roots of pi r^2 = c (1 - l^2) asin(r / (1 - r))
-//   angles       containment of mpmath point evaluations + monotonicity
-//   criteria     point-sample soundness against mpmath + an exact-rational
-//                interval re-implementation on small cell sets
-//   driver       synthetic trace test of the lambda queue rule
+//                (test_oracle_geometry.py)
+//   angles       enclosure of mpmath law-of-cosines values + the reference's
+//                monotonicity claims (test_oracle_geometry.py)
+//   criteria, kernel bodies, ranges, Mode R search and output lines
+//                an exact-rational restatement written from the reference
+//                files (tests/exact_ref.py): verdict bytes, both sides of
+//                every area comparison, range endpoints, first problematic la
+//                and failure rule, printed lines (test_oracle_criteria_exact.py);
+//                mpmath: the area comparisons at point cells enclose the
+//                real-valued terms, PROVEN cells satisfy the inequalities at
+//                interior points of the admissible domain
+//                (test_oracle_criteria_soundness.py); planted mistakes are
+//                caught (test_oracle_mutants.py)
+//   ordered map  brute force over bit patterns (test_ordered_transform.py)
+//   driver       synthetic trace test of the lambda queue rule (test_driver.py)
 #pragma once
 
 #include <cfloat>
@@ -714,6 +727,28 @@ static inline uint8_t vbyte(int v, int nofit_flag, int detail) {
   return (uint8_t)(v | (nofit_flag ? 8 : 0) | (detail << 4));
 }
 
+// Checker instrumentation (off unless armed): records the two sides of every
+// area comparison the criteria make (lower bound of the disk area, upper bound
+// of critical_ratio times the used area), in evaluation order, so tests can
+// compare them with the real-valued terms at point cells.
+struct AreaProbe {
+  int on = 0, n = 0;
+  double v[16];
+};
+inline AreaProbe& area_probe() {
+  static thread_local AreaProbe p;
+  return p;
+}
+static inline bool area_holds(double lb_disk_area, double ub_bound) {
+  AreaProbe& p = area_probe();
+  if (p.on && p.n < 8) {
+    p.v[2 * p.n] = lb_disk_area;
+    p.v[2 * p.n + 1] = ub_bound;
+    ++p.n;
+  }
+  return lb_disk_area >= ub_bound;
+}
+
 // Range functions (a11).  Shared r1 rule: start_configuration.cu:58-66,
 // middle_configuration.cu:66-74, end_configuration.cu:55-63.
 static inline IV compute_r1_range(IV la, double lbrsp_of_la_lb) {
@@ -765,7 +800,7 @@ static inline bool handle_case_1(IV la, IV r1, IV r2, IV a12) {
   IV a = semicircle_angle_outer(r1) + a12;
   double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
   double lb_disk_area = add_rd(lb_area1, mul_rd(0.5, lb_area2));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, ub_area_used));
 }
 static inline bool handle_case_2(IV la, IV r1, IV r2, IV r3, IV a13) {
   double ub_ratio = add_ru(1.0, -mul_rd(la.lb, la.lb));
@@ -775,7 +810,7 @@ static inline bool handle_case_2(IV la, IV r1, IV r2, IV r3, IV a13) {
   IV a = semicircle_angle_outer(r1) + a13;
   double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, a.ub));
   double lb_disk_area = add_rd(add_rd(lb_area1, lb_area2), mul_rd(0.5, lb_area3));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, ub_area_used));
 }
 
 // start_configuration.cu:169-188
@@ -795,7 +830,7 @@ static inline bool handle_2d_case(IV la, IV r1, IV r2) {
   double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_disk_area = add_rd(lb_area2, lb_area1);
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable)));
 }
 
 // start_configuration.cu:227-265 (keeps the reference's outer_outer(r1, r2) at :232)
@@ -826,7 +861,7 @@ static inline bool handle_3d_case(IV la, IV r1, IV r2, IV r3) {
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
   double lb_disk_area = add_rd(lb_area1, add_rd(lb_area2, lb_area3));
-  return lb_disk_area >= mul_ru(critical_ratio, ub_area_used);
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, ub_area_used));
 }
 
 // ---- middle configuration (middle_configuration.cu:86-154) ----
@@ -838,13 +873,13 @@ static inline bool middle_area_checks(IV la, IV r1, IV r2, IV r3, IV c12, IV c13
   if (possibly(case_a)) {
     double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c12.ub));
     double lb_disk_area = mul_rd(0.5, add_rd(lb_area1, lb_area2));
-    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+    if (!area_holds(lb_disk_area, mul_ru(critical_ratio, ub_area_used))) return false;
   }
   if (possibly(!case_a)) {
     double ub_area_used = mul_ru(ub_ratio, mul_ru(0.5, c13.ub));
     double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
     double lb_disk_area = add_rd(mul_rd(0.5, add_rd(lb_area1, lb_area3)), lb_area2);
-    if (lb_disk_area < mul_ru(critical_ratio, ub_area_used)) return false;
+    if (!area_holds(lb_disk_area, mul_ru(critical_ratio, ub_area_used))) return false;
   }
   return true;
 }
@@ -873,7 +908,7 @@ static inline bool handle_inner_outer_2d(IV la, IV r1, IV r2) {
   double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_disk_area = add_rd(lb_area2, mul_rd(0.5, lb_area1));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable)));
 }
 static inline bool handle_outer_inner_2d(IV la, IV r1, IV r2) {
   IV a = collision_angle_outer_inner(la, r1, r2) + semicircle_angle_inner(la, r2);
@@ -886,7 +921,7 @@ static inline bool handle_outer_inner_2d(IV la, IV r1, IV r2) {
   double lb_area1 = mul_rd(lb_pi, mul_rd(r1.lb, r1.lb));
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_disk_area = add_rd(lb_area2, mul_rd(0.5, lb_area1));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable)));
 }
 
 // ---- end configuration, 3 disks (end_configuration.cu:154-210) ----
@@ -908,7 +943,7 @@ static inline bool handle_r1_inner_3d(IV la, IV r1, IV r2, IV r3) {
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
   double lb_disk_area = add_rd(add_rd(lb_area2, lb_area3), mul_rd(0.5, lb_area1));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable)));
 }
 static inline bool handle_r1_outer_3d(IV la, IV r1, IV r2, IV r3) {
   IV a12 = collision_angle_outer_inner(la, r1, r2);
@@ -928,7 +963,7 @@ static inline bool handle_r1_outer_3d(IV la, IV r1, IV r2, IV r3) {
   double lb_area2 = mul_rd(lb_pi, mul_rd(r2.lb, r2.lb));
   double lb_area3 = mul_rd(lb_pi, mul_rd(r3.lb, r3.lb));
   double lb_disk_area = add_rd(add_rd(lb_area2, lb_area3), mul_rd(0.5, lb_area1));
-  return lb_disk_area >= mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable));
+  return area_holds(lb_disk_area, mul_ru(critical_ratio, add_ru(ub_area_used, -lb_area_reusable)));
 }
 
 // Fit check shared by start (print only), middle and end (trap):
@@ -983,6 +1018,57 @@ static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3) {
     return vbyte(TRAP, nofit_flag, t.code);
   }
   return vbyte(TRAP, 0, 15);
+}
+
+// The reference's output_criticals mode on one cell (the kernels keep
+// evaluating after a failing criterion and print one line per failure:
+// start_configuration.cu:142-161, :202-210, :284-292; middle_configuration.cu:180-200;
+// end_configuration.cu:122-142, :243-263).  Returns the bit mask of failing
+// criteria (bit k-1: criterion k) or throws Trap; *nofit_flag (start family):
+// the "possibly does not fit" line of start_configuration.cu:131-135.  Cells
+// whose verdict is decided before the criteria report through `early`
+// (EMPTY, or NOFIT of the families that trap there), with mask 0.
+static inline int eval_kernel_body_all(int f, IV la, IV r1_, IV r2_, IV r3, int* nofit_flag, int* early) {
+  IV r2 = r2_;
+  if (family_is_3d(f)) r2.tighten_lb(r3.lb);
+  IV r1 = r1_;
+  r1.tighten_lb(r2.lb);
+  *nofit_flag = 0;
+  *early = -1;
+  if (r1.empty() || r2.empty()) { *early = EMPTY; return 0; }
+  if (f == START || f == MIDDLE || f == END_2D || f == END_3D) {
+    if (possibly_no_fit(la, r1)) {
+      if (f != START) { *early = NOFIT; return 0; }
+      *nofit_flag = 1;
+    }
+  }
+  int mask = 0;
+  switch (f) {
+    case START: {
+      IV a12 = collision_angle_outer_inner(la, r1, r2);
+      IV a13 = collision_angle_outer_outer(r1, r3);
+      IV a23 = collision_angle_inner_outer(la, r2, r3);
+      UB case1 = (a12 + a23 > a13);
+      if (possibly(case1) && !handle_case_1(la, r1, r2, a12)) mask |= 1;
+      if (possibly(!case1) && !handle_case_2(la, r1, r2, r3, a13)) mask |= 2;
+      break;
+    }
+    case START_END_2D: if (!handle_2d_case(la, r1, r2)) mask |= 1; break;
+    case START_END_3D: if (!handle_3d_case(la, r1, r2, r3)) mask |= 1; break;
+    case MIDDLE:
+      if (!handle_case_1_1(la, r1, r2, r3)) mask |= 1;
+      if (!handle_case_2_1(la, r1, r2, r3)) mask |= 2;
+      break;
+    case END_2D:
+      if (!handle_inner_outer_2d(la, r1, r2)) mask |= 1;
+      if (!handle_outer_inner_2d(la, r1, r2)) mask |= 2;
+      break;
+    case END_3D:
+      if (!handle_r1_inner_3d(la, r1, r2, r3)) mask |= 1;
+      if (!handle_r1_outer_3d(la, r1, r2, r3)) mask |= 2;
+      break;
+  }
+  return mask;
 }
 
 // r3 from a position interval p in [0,1] over the admissible r3 range; for a

@@ -6,6 +6,7 @@
 #include "../include/cpl.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -230,6 +231,17 @@ int cplo_prim(int op, const double* a, const double* b, double* out, size_t n) {
       case 35: r = dcos_ru(x); break;
       case 36: r = dacos_rd(x); break;
       case 37: r = dacos_ru(x); break;
+      case 38: {  // Atomic_ordered_double::transform, the u64 carried in the double's bits
+        uint64_t o = ordered_from_double(x);
+        std::memcpy(&r, &o, 8);
+        break;
+      }
+      case 39: {  // inverse transform of the u64 carried in x's bits
+        uint64_t o;
+        std::memcpy(&o, &x, 8);
+        r = double_from_ordered(o);
+        break;
+      }
       default: return CPL_ERR_ARG;
     }
     out[i] = r;
@@ -304,6 +316,50 @@ int cplo_geom(int fn, const double* in, double* out, int* trap, size_t n) {
     out[2 * i] = r.lb;
     out[2 * i + 1] = r.ub;
     if (trap) trap[i] = t;
+  }
+  return CPL_OK;
+}
+
+// Admissible ranges of family f (a checker for the range functions): in: n
+// records of 6 doubles (la, r1, r2 intervals); out: n x 6 doubles =
+// compute_r1_range(la), r2 range(la, r1), r3 range(la, r1, r2) (the r3 range of
+// the 3-d families; zeros for the 2-d ones).
+int cplo_ranges(int family, const double* in, double* out, size_t n) {
+  if (family < 0 || family >= NUM_FAMILIES) return CPL_ERR_ARG;
+  for (size_t i = 0; i < n; ++i) {
+    const double* p = in + 6 * i;
+    IV la(p[0], p[1]), r1(p[2], p[3]), r2(p[4], p[5]);
+    IV a = compute_r1_range(la, lower_bound_radius_single_placement(la.lb));
+    IV b = r2_range_for(family, la, r1);
+    IV c = family_is_3d(family) ? r3_range_for(family, la, r1, r2) : IV(0.0, 0.0);
+    double* o = out + 6 * i;
+    o[0] = a.lb; o[1] = a.ub; o[2] = b.lb; o[3] = b.ub; o[4] = c.lb; o[5] = c.ub;
+  }
+  return CPL_OK;
+}
+
+// The area comparisons of the output_criticals evaluation of n cells (8
+// doubles each, as cplo_eval_kernel_body): out n x 16 doubles = up to 8 pairs
+// (lower bound of the disk area, upper bound of critical_ratio x used area) in
+// evaluation order; count n ints = pairs recorded, or -1 when the cell is
+// decided before the criteria or traps.
+int cplo_area_terms(int family, const double* cells, size_t n, double* out, int* count) {
+  if (family < 0 || family >= NUM_FAMILIES) return CPL_ERR_ARG;
+  AreaProbe& p = area_probe();
+  for (size_t i = 0; i < n; ++i) {
+    const double* c = cells + 8 * i;
+    p.on = 1;
+    p.n = 0;
+    int nofit = 0, early = -1, k = 0;
+    try {
+      eval_kernel_body_all(family, IV(c[0], c[1]), IV(c[2], c[3]), IV(c[4], c[5]), IV(c[6], c[7]), &nofit, &early);
+      k = early >= 0 ? -1 : p.n;
+    } catch (const Trap&) {
+      k = -1;
+    }
+    p.on = 0;
+    for (int j = 0; j < 16; ++j) out[16 * i + j] = j < 2 * p.n ? p.v[j] : 0.0;
+    count[i] = k;
   }
   return CPL_OK;
 }
@@ -410,26 +466,51 @@ int cplo_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t*
 }
 
 // Reference-grid search (Mode R) for one lambda interval, evaluating every
-// cell: first = min la.lb over critical cells (DBL_MAX if none).  A trap or
-// a middle/end fit failure anywhere returns CPL_ERR_TRAP / CPL_ERR_NOFIT.
+// cell of the grid (start_configuration.cu:114-167, :190-215, :267-298;
+// middle_configuration.cu:156-205; end_configuration.cu:103-146, :212-268).
+//   first: min la.lb over cells with a failing criterion (DBL_MAX if none).
+//   A trap, or the fit check of the middle and end families, is reported only
+//   where the reference is bound to reach it: every thread of the reference
+//   walks its cells in increasing la and returns at its first failing cell, so
+//   every cell with la.lb < first is reached, while cells at or above first may
+//   or may not be (that depends on the thread mapping).  rc = CPL_ERR_TRAP when
+//   a trapping cell has la.lb < first (trap_code: the code whose smallest
+//   trapping la.lb is smallest, then the smallest code), else CPL_ERR_NOFIT
+//   when a fit-check cell has la.lb < first.  With output_criticals the
+//   reference evaluates every cell and every criterion: every trap and fit
+//   failure counts, and recs receives one record per printed line (9 doubles:
+//   kind, la, r1, r2, r3 as printed; kind 1/2 = failing criterion 1/2, 3 = the
+//   start family's "possibly does not fit" line), sorted, at most rec_cap of
+//   the n_recs_out found.  nofit_flags_out: cells of the start family whose
+//   fit check fired (the reference prints a line for each it reaches).
 int cplo_first_problematic(int family, double outer_lb, double outer_ub, const int32_t* subdiv,
-                           double* first_out, uint64_t* evaluated_out, int* trap_code_out, int nthreads) {
-  if (family < 0 || family >= NUM_FAMILIES || !subdiv) return CPL_ERR_ARG;
+                           int output_criticals, double* first_out, uint64_t* evaluated_out, int* trap_code_out,
+                           double* recs, uint64_t rec_cap, uint64_t* n_recs_out, uint64_t* nofit_flags_out,
+                           int nthreads) {
+  if (family < 0 || family >= NUM_FAMILIES || !subdiv || !first_out) return CPL_ERR_ARG;
   const int nla = subdiv[0], nr1 = subdiv[1], nr2 = subdiv[2];
   const int nr3 = family_is_3d(family) ? subdiv[3] : 1;
+  if (nla < 1 || nr1 < 1 || nr2 < 1 || nr3 < 1) return CPL_ERR_ARG;
   IV outer(outer_lb, outer_ub);
   int T = hw_threads(nthreads);
   std::atomic<int> next{0};
-  std::atomic<uint64_t> best{ordered_from_double(DBL_MAX)};
-  std::atomic<uint64_t> evaluated{0};
-  std::atomic<int> nofit{0}, trap{0};
+  std::mutex mu;
+  uint64_t best = ordered_from_double(DBL_MAX);
+  uint64_t fail_min[6];  // [0] fit check, [c] trap code c
+  for (uint64_t& v : fail_min) v = ~0ull;
+  uint64_t evaluated = 0, nofit_flags = 0;
+  std::vector<std::array<double, 9>> lines;
   auto work = [&]() {
-    uint64_t ev = 0;
+    uint64_t ev = 0, nf = 0, b = ordered_from_double(DBL_MAX);
+    uint64_t fm[6];
+    for (uint64_t& v : fm) v = ~0ull;
+    std::vector<std::array<double, 9>> mine;
     for (;;) {
       int job = next.fetch_add(1);
       if (job >= nla * nr1) break;
       int ia = job / nr1, i1 = job % nr1;
       IV la = get_subinterval(outer, ia, nla);
+      const uint64_t o = ordered_from_double(la.lb);
       IV r1_range = compute_r1_range(la, lower_bound_radius_single_placement(la.lb));
       IV r1_ = get_subinterval(r1_range, i1, nr1);
       IV r2_range = r2_range_for(family, la, r1_);
@@ -442,34 +523,66 @@ int cplo_first_problematic(int family, double outer_lb, double outer_ub, const i
         }
         for (int i3 = 0; i3 < nr3; ++i3) {
           IV r3 = family_is_3d(family) ? get_subinterval(r3_range, i3, nr3) : IV(0, 0);
-          uint8_t vb = eval_kernel_body(family, la, r1_, r2_, r3);
           ++ev;
-          int v = vb & 7;
-          if (v == CRITICAL) {
-            uint64_t o = ordered_from_double(la.lb);
-            uint64_t cur = best.load();
-            while (o < cur && !best.compare_exchange_weak(cur, o)) {
-            }
-          } else if (v == NOFIT) {
-            nofit = 1;
-          } else if (v == TRAP) {
-            int expected = 0;
-            trap.compare_exchange_strong(expected, vb >> 4);
+          if (!output_criticals) {
+            uint8_t vb = eval_kernel_body(family, la, r1_, r2_, r3);
+            int v = vb & 7;
+            if (vb & 8) ++nf;
+            if (v == CRITICAL) b = std::min(b, o);
+            else if (v == NOFIT) fm[0] = std::min(fm[0], o);
+            else if (v == TRAP) fm[vb >> 4] = std::min(fm[vb >> 4], o);
+            continue;
           }
+          int nofit_flag = 0, early = -1, mask = 0;
+          try {
+            mask = eval_kernel_body_all(family, la, r1_, r2_, r3, &nofit_flag, &early);
+          } catch (const Trap& t) {
+            fm[t.code] = std::min(fm[t.code], o);
+          }
+          if (early == NOFIT) fm[0] = std::min(fm[0], o);
+          if (early >= 0) continue;
+          IV r2 = r2_;
+          if (family_is_3d(family)) r2.tighten_lb(r3.lb);
+          IV r1 = r1_;
+          r1.tighten_lb(r2.lb);
+          auto line = [&](int kind) {
+            mine.push_back({double(kind), la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, r3.lb, r3.ub});
+          };
+          if (nofit_flag) {
+            ++nf;
+            line(3);
+          }
+          if (mask) b = std::min(b, o);
+          for (int k = 0; k < 2; ++k)
+            if (mask & (1 << k)) line(k + 1);
         }
       }
     }
+    std::lock_guard<std::mutex> g(mu);
     evaluated += ev;
+    nofit_flags += nf;
+    best = std::min(best, b);
+    for (int k = 0; k < 6; ++k) fail_min[k] = std::min(fail_min[k], fm[k]);
+    lines.insert(lines.end(), mine.begin(), mine.end());
   };
   std::vector<std::thread> th;
   for (int t = 1; t < T; ++t) th.emplace_back(work);
   work();
   for (auto& t : th) t.join();
-  *first_out = double_from_ordered(best.load());
-  if (evaluated_out) *evaluated_out = evaluated.load();
-  if (trap_code_out) *trap_code_out = trap.load();
-  if (trap.load()) return CPL_ERR_TRAP;
-  if (nofit.load()) return CPL_ERR_NOFIT;
+  *first_out = double_from_ordered(best);
+  if (evaluated_out) *evaluated_out = evaluated;
+  if (nofit_flags_out) *nofit_flags_out = nofit_flags;
+  std::sort(lines.begin(), lines.end());
+  if (n_recs_out) *n_recs_out = lines.size();
+  if (recs)
+    for (size_t k = 0; k < lines.size() && k < rec_cap; ++k) std::memcpy(recs + 9 * k, lines[k].data(), 9 * 8);
+  const uint64_t threshold = output_criticals ? ~0ull : best;
+  int code = 0;
+  for (int c = 1; c < 6; ++c)
+    if (fail_min[c] < threshold && (code == 0 || fail_min[c] < fail_min[code])) code = c;
+  if (trap_code_out) *trap_code_out = code;
+  if (code) return CPL_ERR_TRAP;
+  if (fail_min[0] < threshold) return CPL_ERR_NOFIT;
   return CPL_OK;
 }
 

@@ -35,14 +35,18 @@ def lib(counting=False):
         L.cplo_geom.argtypes = [ctypes.c_int, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
         L.cplo_interval_op.argtypes = [ctypes.c_int, _P_D, _P_D, _P_D, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t]
         L.cplo_eval_kernel_body.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_U8]
+        L.cplo_ranges.argtypes = [ctypes.c_int, _P_D, _P_D, ctypes.c_size_t]
+        L.cplo_area_terms.argtypes = [ctypes.c_int, _P_D, ctypes.c_size_t, _P_D, ctypes.POINTER(ctypes.c_int)]
         L.cplo_eval_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P_D), ctypes.c_size_t, _P_U8,
                                       ctypes.c_int]
         L.cplo_refine_run.argtypes = [ctypes.POINTER(abi.RefineCfg), ctypes.POINTER(abi.RefineStats), _P_U8,
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_uint64)]
         L.cplo_first_problematic.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                             ctypes.POINTER(ctypes.c_int32), _P_D, ctypes.POINTER(ctypes.c_uint64),
-                                             ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.c_int, _P_D,
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int), _P_D,
+                                             ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
         _libs[path] = L
     return _libs[path]
 
@@ -83,6 +87,26 @@ def eval_kernel_body(family, cells):
     return out
 
 
+def area_terms(family, cells, library=None):
+    """-> ((n, 8, 2) area comparisons (disk-area lower bound, bound on c x used
+    area) of the output_criticals evaluation, (n,) counts; -1: no criteria ran)."""
+    cells = np.ascontiguousarray(cells, dtype=np.float64)
+    n = cells.shape[0]
+    out = np.zeros((n, 16))
+    cnt = np.zeros(n, dtype=np.int32)
+    L = library or lib()
+    assert L.cplo_area_terms(family, _dptr(cells), n, _dptr(out), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) == 0
+    return out.reshape(n, 8, 2), cnt
+
+
+def ranges(family, recs):
+    """recs: (n, 6) = la, r1, r2 intervals -> (n, 6) = r1 range(la), r2 range(la, r1), r3 range(la, r1, r2)."""
+    recs = np.ascontiguousarray(recs, dtype=np.float64)
+    out = np.empty_like(recs)
+    assert lib().cplo_ranges(family, _dptr(recs), _dptr(out), recs.shape[0]) == 0
+    return out
+
+
 def eval_cells(family, soa, clip=True, nthreads=0):
     soa = np.ascontiguousarray(soa, dtype=np.float64)
     n = soa.shape[1]
@@ -114,14 +138,31 @@ def refine_run(cfg, want_flags=True, root_begin=0, root_end=0, root_stride=1, nt
 
 
 def first_problematic(family, outer_la, subdiv, nthreads=0):
-    """-> (rc, first, evaluated, trap_code)"""
+    """-> (rc, first, evaluated, trap_code); every cell of the grid is evaluated."""
+    r = mode_r(family, outer_la, subdiv, nthreads=nthreads)
+    return r["rc"], r["first"], r["evaluated"], r["trap_code"]
+
+
+def mode_r(family, outer_la, subdiv, output_criticals=False, rec_cap=1 << 16, nthreads=0):
+    """One reference-grid launch of the oracle as a dict: rc, first, evaluated,
+    trap_code, nofit_flags and, with output_criticals, records (k, 9) =
+    kind, la, r1, r2, r3 of each printed line (sorted) and n_records."""
     first = ctypes.c_double()
     ev = ctypes.c_uint64()
     trap = ctypes.c_int()
+    nrec = ctypes.c_uint64()
+    nof = ctypes.c_uint64()
     sd = (ctypes.c_int32 * 4)(*subdiv)
-    rc = lib().cplo_first_problematic(family, outer_la[0], outer_la[1], sd, ctypes.byref(first), ctypes.byref(ev),
-                                      ctypes.byref(trap), nthreads)
-    return rc, first.value, ev.value, trap.value
+    recs = np.zeros((rec_cap if output_criticals else 0, 9))
+    rc = lib().cplo_first_problematic(family, outer_la[0], outer_la[1], sd, int(bool(output_criticals)),
+                                      ctypes.byref(first), ctypes.byref(ev), ctypes.byref(trap),
+                                      _dptr(recs) if output_criticals else None, recs.shape[0], ctypes.byref(nrec),
+                                      ctypes.byref(nof), nthreads)
+    out = {"rc": rc, "first": first.value, "evaluated": ev.value, "trap_code": trap.value,
+           "nofit_flags": nof.value, "n_records": nrec.value}
+    if output_criticals:
+        out["records"] = recs[:min(nrec.value, rec_cap)]
+    return out
 
 
 def interval_op(op, a, b=None):

@@ -2,6 +2,7 @@
 import numpy as np
 
 from circlepacking_amd import abi
+from tests import exact_ref
 
 
 def c1_grid_soa(n=256, lam=abi.LAMBDA_MIN):
@@ -52,3 +53,25 @@ def random_kernel_body_cells(seed, n, family):
     out[np.nonzero(m)[0], 5] = r[:, 1] + rw[:, 1]
     out[np.nonzero(m)[0], 7] = r[:, 2] + rw[:, 2]
     return out
+
+
+def grid_cells(ref, f, n, seed, subdiv):
+    """n random cells (la, r1_, r2_, r3; before tightening) of the reference's
+    nested grid with the given subdivision counts (subintervals.hpp:31-42 and
+    the families' range functions), lambda over [lambda_min, lambda_max]."""
+    rng = np.random.default_rng(seed)
+    nla, n1, n2, n3 = subdiv
+    out = []
+    while len(out) < n:
+        ia, i1, i2, i3 = (int(rng.integers(k)) for k in (nla, n1, n2, n3))
+        la = exact_ref.get_subinterval(exact_ref.IV(exact_ref.LAMBDA_MIN, exact_ref.LAMBDA_MAX), ia, nla)
+        r1_ = exact_ref.get_subinterval(ref.compute_r1_range(la), i1, n1)
+        r2_ = exact_ref.get_subinterval(ref.compute_r2_range(f, la, r1_), i2, n2)
+        r3 = exact_ref.IV(0.0, 0.0)
+        if f in exact_ref.THREE_D:
+            r3_range = ref.compute_r3_range(f, la, r1_, r2_)
+            if f == exact_ref.END_3D and r3_range.empty():
+                continue
+            r3 = exact_ref.get_subinterval(r3_range, i3, n3)
+        out.append([la.lb, la.ub, r1_.lb, r1_.ub, r2_.lb, r2_.ub, r3.lb, r3.ub])
+    return np.array(out)

@@ -9,7 +9,9 @@
 #include "../circlepacking_amd/csrc/run_until_no_criticals.hpp"
 
 extern "C" int cplo_first_problematic(int family, double outer_lb, double outer_ub, const int* subdiv,
-                                      double* first_out, unsigned long long* evaluated_out, int* trap_code_out,
+                                      int output_criticals, double* first_out, unsigned long long* evaluated_out,
+                                      int* trap_code_out, double* recs, unsigned long long rec_cap,
+                                      unsigned long long* n_recs_out, unsigned long long* nofit_flags_out,
                                       int nthreads);
 
 int main(int argc, char** argv) {
@@ -26,7 +28,7 @@ int main(int argc, char** argv) {
     double first = DBL_MAX;
     unsigned long long ev = 0;
     int trap = 0;
-    int rc = cplo_first_problematic(family, la.lb, la.ub, sd, &first, &ev, &trap, 0);
+    int rc = cplo_first_problematic(family, la.lb, la.ub, sd, 0, &first, &ev, &trap, nullptr, 0, nullptr, nullptr, 0);
     if (rc != 0) {
       std::cerr << "error " << rc << std::endl;
       std::exit(4);
