@@ -82,6 +82,29 @@ def width_ratio_histogram(cpl, n=20000, seed=0):
     }
 
 
+def reference_grid_launches(cpl, abi):
+    """The reference's own kernels: one launch per family over [lambda_min,
+    lambda_max] on its full grids (cpl_first_problematic_ex, the call its prover
+    makes per lambda interval), every cell evaluated (cells/s), then the same
+    launch with the early exit the reference's threads take."""
+    reference_grid = {}
+    lam = (abi.LAMBDA_MIN, abi.LAMBDA_MAX)
+    cpl.first_problematic_ex(0, lam, exhaustive=True)  # warm-up
+    for f in range(6):
+        t0 = time.perf_counter()
+        first, info, _ = cpl.first_problematic_ex(f, lam, exhaustive=True)
+        dt = time.perf_counter() - t0
+        ev = info["cells_evaluated"]
+        t0 = time.perf_counter()
+        first_x, info_x, _ = cpl.first_problematic_ex(f, lam)
+        dt_x = time.perf_counter() - t0
+        reference_grid[abi.FAMILY_NAMES[f]] = {
+            "ms": dt * 1e3, "cells": ev, "cells_per_s": ev / dt, "first_problematic_la": first,
+            "early_exit": {"ms": dt_x * 1e3, "cells_evaluated": info_x["cells_evaluated"],
+                           "rows_skipped": info_x["rows_skipped"], "same_first": first_x == first}}
+    return reference_grid
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +113,8 @@ def main():
     ap.add_argument("--variant", default="tuned", choices=["plain", "tuned"],
                     help="which form `value` reports; every form is timed and checked for identical results")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (256 threads, 1 cell/thread)")
+    ap.add_argument("--graph-shape", default=None,
+                    help="block,cells_per_thread of the graph form instead of the autotuner's (e.g. 64,2)")
     ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned proposal")
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--depth", type=int, default=6)
@@ -100,6 +125,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-reference-grid", action="store_true",
                     help="skip the one-launch-per-family timing on the reference grids")
+    ap.add_argument("--reference-grid-only", action="store_true",
+                    help="only the one-launch-per-family timing on the reference grids (for rocprofv3), then exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 256 MiB device copy three times and one plain run, then exit (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -122,6 +149,9 @@ def main():
         cpl.refine_run(abi.with_options(abi.cfg_4param(n=args.grid, max_depth=args.depth), abi.OPT_NO_DIGEST),
                        want_flags=False)
         print(json.dumps({"traffic_probe": True, "copy_bytes": 4 * a.numel()}), flush=True)
+        return
+    if args.reference_grid_only:
+        print(json.dumps({"reference_grid_launch": reference_grid_launches(cpl, abi)}), flush=True)
         return
     want = set(args.forms.split(","))
 
@@ -152,7 +182,10 @@ def main():
     graph_cfg = abi.with_options(base, G | H | ND)
     persist_cfg = abi.with_options(base, PER | H | ND, block=64, cpt=1)
     tune_graph = tune_persist = None
-    if not args.no_tune and "graph" in want:
+    if args.graph_shape:
+        b, c = (int(x) for x in args.graph_shape.split(","))
+        graph_cfg = abi.with_options(graph_cfg, graph_cfg.options, block=b, cpt=c)
+    elif not args.no_tune and "graph" in want:
         tune_graph = cpl.autotune(graph_cfg, reps=3)
     if not args.no_tune and "persistent" in want:
         tune_persist = cpl.autotune(persist_cfg, reps=3)
@@ -210,26 +243,34 @@ def main():
         name = tuned_name
     ms_per_step = forms[name]["ms"]
     certified = st_plain.certified
+    proven_by_criteria = sum(st_plain.verdicts[d][abi.V_PROVEN] for d in range(st_plain.rounds))
+    criteria_evaluations = sum(st_plain.verdicts[d][v] for d in range(st_plain.rounds)
+                               for v in (abi.V_PROVEN, abi.V_CRITICAL, abi.V_TRAP))
     value = certified / (ms_per_step * 1e-3)
 
     # ---- CPU oracle on a stride sample: baseline timing + per-evaluation flop counts ----
     cpu = None
+    cpu_parity = None
     flops_per_eval = None
     if not args.no_cpu:
         from oracle import cplo
 
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         c0 = time.perf_counter()
-        ost, _ = cplo.refine_run(base, want_flags=False, root_stride=args.cpu_stride, nthreads=threads)
+        ost, oflags = cplo.refine_run(base, want_flags=True, root_stride=args.cpu_stride, nthreads=threads)
         c1 = time.perf_counter()
         cpu = {
             "value": ost.certified / (c1 - c0),
             "unit": "cells/s",
             "cores": threads,
+            "host_cores": os.cpu_count(),
             "kind": "port",
             "sample": f"every {args.cpu_stride}th root cell of the same {args.grid}^4 grid ({ost.total_evaluations} "
                       f"evaluations, {ost.certified} certified leaves, {c1 - c0:.1f} s wall)",
         }
+        # the same sample is a parity check of the GPU's per-(root, family) outcome flags
+        cpu_parity = {"roots_compared": int(oflags.shape[0]),
+                      "identical": bool(np.array_equal(oflags, plain_ref[1][::args.cpu_stride]))}
         fst, _, flops = cplo.refine_run(base, want_flags=False, root_stride=max(args.cpu_stride, 97),
                                         nthreads=threads, counting=True)
         counts = np.zeros((6, 8))
@@ -260,8 +301,8 @@ def main():
                 t = json.load(fh)
             traffic = t.get(kname)
         roof = {
-            "bound": "mfma",
-            "pipe": "FP64 VALU (directed-rounding interval arithmetic; published FP64 peak)",
+            "bound": "fp64_valu",
+            "pipe": "FP64 VALU (directed-rounding interval arithmetic; published FP64 vector peak)",
             "kernel": kname,
             "family": abi.FAMILY_NAMES[f],
             "launches": len(launches),
@@ -276,19 +317,7 @@ def main():
             "worklist_items_per_launch": items,
         }
 
-    # ---- the reference's own kernels: one launch per family over [lambda_min, lambda_max] on its
-    # full grids (cpl_first_problematic, the call its prover makes per lambda interval) ----
-    reference_grid = None
-    if not args.no_reference_grid:
-        reference_grid = {}
-        lam = (abi.LAMBDA_MIN, abi.LAMBDA_MAX)
-        cpl.first_problematic(0, lam)  # warm-up
-        for f in range(6):
-            t0 = time.perf_counter()
-            first, ev, _ = cpl.first_problematic(f, lam)
-            dt = time.perf_counter() - t0
-            reference_grid[abi.FAMILY_NAMES[f]] = {"ms": dt * 1e3, "cells": ev, "cells_per_s": ev / dt,
-                                                   "first_problematic_la": first}
+    reference_grid = None if args.no_reference_grid else reference_grid_launches(cpl, abi)
 
     out = {
         "metric": METRIC,
@@ -323,9 +352,15 @@ def main():
         "evaluations": st_plain.total_evaluations,
         "evaluations_per_s": st_plain.total_evaluations / (ms_per_step * 1e-3),
         "certified": certified,
+        "certified_note": "leaves settled PROVEN, EMPTY or OOD (out-of-domain clipping included)",
+        "proven_by_criteria": proven_by_criteria,
+        "proven_by_criteria_per_s": proven_by_criteria / (ms_per_step * 1e-3),
+        "criteria_evaluations": criteria_evaluations,
+        "criteria_evaluations_per_s": criteria_evaluations / (ms_per_step * 1e-3),
         "unresolved": st_plain.unresolved,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_parity": cpu_parity,
         "reference_grid_launch": reference_grid,
     }
     print(json.dumps(out), flush=True)

@@ -73,6 +73,20 @@ class Iv(ctypes.Structure):
     _fields_ = [("lb", ctypes.c_double), ("ub", ctypes.c_double)]
 
 
+MR_OUTPUT_CRITICALS, MR_EXHAUSTIVE = 1, 2
+MR_MAX_RECORDS = 1 << 20
+
+
+class ModeRInfo(ctypes.Structure):
+    _fields_ = [("cells_evaluated", ctypes.c_uint64), ("rows_skipped", ctypes.c_uint64),
+                ("nofit_flags", ctypes.c_uint64), ("lines_found", ctypes.c_uint64),
+                ("lines_written", ctypes.c_uint64), ("text_bytes_needed", ctypes.c_uint64),
+                ("trap_code", ctypes.c_int32), ("fit_check_failed", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class RefineCfg(ctypes.Structure):
     _fields_ = [
         ("grid", ctypes.c_int32 * 4),

@@ -952,16 +952,20 @@ __global__ __launch_bounds__(256) void k_body(const double* cells, size_t n, uin
 }
 
 // ---- reference grid (Mode R) ----------------------------------------------
+// Scratch of one launch (device memory, MODE_R_MISC_BYTES): the ordered-u64
+// minimum of failing la.lb (Atomic_ordered_double), the smallest la.lb of each
+// failure kind, the record counter, and 64 statistics slots of one 128-byte
+// line each (per wave: cells evaluated, rows skipped, start no-fit flags).
+enum : int { MR_FIRST = 0, MR_FAIL = 4, MR_NREC = 10, MR_SLOTS = 32, MR_SLOT_STRIDE = 16, MR_NSLOTS = 64 };
+constexpr size_t MODE_R_MISC_BYTES = (MR_SLOTS + MR_NSLOTS * MR_SLOT_STRIDE) * 8;
+
 struct ModeRParams {
   IV outer;
   int n[4];                   // subdivisions la, r1, r2, r3 (r3 = 1 for 2-d families)
   const double* r1_ranges;    // per la index: la.lb, la.ub, r1_range.lb, r1_range.ub
-  unsigned long long* first;  // ordered-u64 minimum of failing la.lb (Atomic_ordered_double)
-  unsigned long long* evaluated;
-  uint32_t* status;           // [0] trap code (first), [1] nofit seen
-  int output_criticals;
-  double* crit_out;           // records: family-case, la, r1, r2, r3 (9 doubles)
-  uint32_t* crit_count;
+  unsigned long long* misc;   // MODE_R_MISC_BYTES, laid out as MR_*
+  int exhaustive;             // evaluate every cell (no early exit)
+  double* crit_out;           // output mode: records of 9 doubles (kind, la, r1, r2, r3)
   uint32_t crit_cap;
 };
 
@@ -976,83 +980,160 @@ __global__ void k_mode_r_ranges(IV outer, int nla, double* out) {
   out[4 * i + 3] = r.ub;
 }
 
-// The grid in the reference's cell order (i3 fastest, then i2, i1, la), walked
-// as rows: a block takes one (la, i1) row at a time (grid stride over rows) and
-// its threads step through the row's n2 * n3 cells 256 apart with a mixed-radix
-// counter, so a cell costs no 64-bit division and the row's r1 subinterval,
-// r2 range and semicircle angle of r1 are computed once per row.  The lanes of
-// a wave hold consecutive cells, as with one flat index per thread.  Four waves
-// per SIMD (<= 128 VGPRs, a few spills outside the criteria's hot path) beat
-// three by 6-13 % on every family; five lose again.
-template <int F>
+// Clears a launch's scratch (a kernel, not a memset, so the call allocates and
+// records nothing but kernels): first = DBL_MAX (run_kernel.cuh:81), failure
+// minima = none, counters = 0.
+__global__ void k_mode_r_init(unsigned long long* misc) {
+  for (int i = threadIdx.x; i < (int)(MODE_R_MISC_BYTES / 8); i += blockDim.x) {
+    unsigned long long v = 0;
+    if (i == MR_FIRST) v = ordered_of(DBL_MAX);
+    else if (i >= MR_FAIL && i < MR_FAIL + 6) v = ~0ull;
+    misc[i] = v;
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One launch of the reference kernel of family F over the reference grid, in
+// the reference's cell order (r3 fastest, then r2, r1, la).  Each wave takes
+// (la, r1) rows in turn (grid stride over rows by the global wave index, so
+// the whole grid sweeps la upwards): it computes the row's r1 subinterval, r2
+// range and semicircle angle of r1 once, and its lanes walk the row's n2 * n3
+// cells 64 apart with a mixed-radix (i2, i3) counter.
+//
+// Without output (the reference's store_min + return, start_configuration.cu:
+// 142-145 and every kernel alike): a reference thread walks its cells in
+// increasing la and returns at its first failing cell, so cells whose la.lb is
+// at or above a failing la.lb already found cannot change the minimum.  A wave
+// skips such rows (the global minimum is re-read every row and every 16 cells
+// of a long row with a relaxed load; a stale value only skips less) and stops
+// a row once one of its lanes failed in it.  Failures other than a failing
+// criterion (trap codes, the middle and end families' fit check) are recorded
+// with their la.lb; the host reports those below the final minimum, the cells
+// every reference thread is bound to reach.  P.exhaustive turns the skipping
+// off (every cell evaluated, as the counts of parity tests need).
+//
+// OUT (output_criticals): every cell and every criterion is evaluated and one
+// record is written per printed line (kind 1/2: failing criterion, 3: the start
+// family's no-fit line); every failure counts.
+template <int F, bool OUT>
 __global__ __launch_bounds__(256, 4) void k_mode_r(ModeRParams P) {
   const uint32_t n1 = (uint32_t)P.n[1], n2 = (uint32_t)P.n[2], n3 = (uint32_t)P.n[3];
   const uint64_t rows = (uint64_t)P.n[0] * n1;
-  // start cell and step of this thread within a row, as (i2, i3) digits
-  const uint32_t s2 = threadIdx.x / n3, s3 = threadIdx.x % n3;
-  const uint32_t d2 = blockDim.x / n3, d3 = blockDim.x % n3;
-  unsigned long long best = ~0ull, ev = 0;
-  for (uint64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // start cell and step of this lane within a row, as (i2, i3) digits
+  const uint32_t s2 = lane / n3, s3 = lane % n3;
+  const uint32_t d2 = 64 / n3, d3 = 64 % n3;
+  unsigned long long* first = P.misc + MR_FIRST;
+  unsigned long long best = ~0ull;       // this lane's smallest failing ordered la.lb
+  unsigned long long published = ~0ull;  // the wave's smallest value published to *first
+  unsigned long long ev = 0, skipped = 0, nofit_flags = 0;
+  for (uint64_t row = wave; row < rows; row += waves) {
     const int ia = (int)(row / n1), i1 = (int)(row % n1);
     const double* rr = P.r1_ranges + 4 * ia;
     const IV la(rr[0], rr[1]);
+    const unsigned long long key = ordered_of(la.lb);
+    if (!OUT && !P.exhaustive) {
+      const unsigned long long g = __hip_atomic_load(first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__ballot(key >= (g < published ? g : published))) {
+        skipped = (rows - 1 - row) / waves + 1;  // this row and the later ones of the wave: la only grows
+        break;
+      }
+    }
     const IV r1_ = get_subinterval(IV(rr[2], rr[3]), i1, P.n[1]);
     const IV r2rng = r2_range<F>(la, r1_);
     // the row's semicircle angle of r1 for the criteria that use it on most cells
     // (the start family's criteria reach it rarely; holding it there costs more)
-    constexpr bool kA1 = F == F_START_END_2D || F == F_START_END_3D || F == F_END_2D;
+    constexpr bool kA1 = !OUT && (F == F_START_END_2D || F == F_START_END_3D || F == F_END_2D);
     IV a1r(0.0, 0.0);
     if constexpr (kA1) a1r = semicircle_angle_outer(r1_);
-    for (uint32_t i2 = s2, i3 = s3; i2 < n2;) {
-      const IV r2_ = get_subinterval(r2rng, (int)i2, P.n[2]);
-      IV r3(0.0, 0.0);
-      bool skip = false;
-      if constexpr (is_3d(F)) {
-        IV range = r3_range<F>(la, r2_);
-        if (F == F_END_3D && range.empty()) skip = true;  // end_configuration.cu:222-224
-        else r3 = get_subinterval(range, (int)i3, P.n[3]);
+    uint32_t iter = 0;
+    for (uint32_t i2 = s2, i3 = s3; __ballot(i2 < n2); ++iter) {
+      if (!OUT && !P.exhaustive) {
+        // the rest of the row cannot lower the minimum once a lane failed in it
+        if (__ballot(best <= key)) break;
+        if (iter && (iter & 15) == 0) {
+          const unsigned long long g = __hip_atomic_load(first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (__ballot(key >= g)) break;
+        }
       }
-      // next cell of this thread (i3 + d3 < 2 n3, one carry at most)
+      const bool active = i2 < n2;
+      const uint32_t c2 = i2, c3 = i3;
+      // next cell of this lane (i3 + d3 < 2 n3, one carry at most)
       i3 += d3;
       i2 += d2;
       if (i3 >= n3) { i3 -= n3; ++i2; }
-      if (skip) continue;
-      uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3, true, nullptr, nullptr, kA1 ? &a1r : nullptr);
+      if (!active) continue;
+      const IV r2_ = get_subinterval(r2rng, (int)c2, P.n[2]);
+      IV r3(0.0, 0.0);
+      if constexpr (is_3d(F)) {
+        IV range = r3_range<F>(la, r2_);
+        if (F == F_END_3D && range.empty()) continue;  // end_configuration.cu:222-224
+        r3 = get_subinterval(range, (int)c3, P.n[3]);
+      }
       ++ev;
-      int v = vb & 7;
-      if (v == V_CRITICAL) {
-        unsigned long long o = ordered_of(la.lb);
-        best = o < best ? o : best;
-        if (P.output_criticals) {
-          uint32_t slot = atomicAdd(P.crit_count, 1u);
-          if (slot < P.crit_cap) {
-            double* rec = P.crit_out + 9 * (size_t)slot;
-            IV r2 = r2_;
-            if (is_3d(F)) r2.tighten_lb(r3.lb);
-            IV r1 = r1_;
-            r1.tighten_lb(r2.lb);
-            rec[0] = (double)(vb >> 4);
-            rec[1] = la.lb; rec[2] = la.ub; rec[3] = r1.lb; rec[4] = r1.ub;
-            rec[5] = r2.lb; rec[6] = r2.ub; rec[7] = r3.lb; rec[8] = r3.ub;
+      if constexpr (!OUT) {
+        const uint8_t vb = eval_kernel_body<F>(la, r1_, r2_, r3, true, nullptr, nullptr, kA1 ? &a1r : nullptr);
+        const int v = vb & 7;
+        nofit_flags += (vb >> 3) & 1;
+        if (v == V_CRITICAL) {
+          best = key < best ? key : best;
+        } else if (v == V_NOFIT || v == V_TRAP) {
+          atomicMin(P.misc + MR_FAIL + (v == V_NOFIT ? 0 : min((int)(vb >> 4), 5)), key);
+        }
+      } else {
+        int nofit = 0, early = -1;
+        const int res = eval_kernel_body_all<F>(la, r1_, r2_, r3, &nofit, &early);
+        if (early == V_NOFIT) atomicMin(P.misc + MR_FAIL, key);
+        if (early >= 0) continue;
+        if (res >= 16) atomicMin(P.misc + MR_FAIL + min(res - 16, 5), key);
+        const int mask = res >= 16 ? 0 : res;
+        if (mask) best = key < best ? key : best;
+        nofit_flags += nofit;
+        if (nofit | mask) {
+          IV r2 = r2_;
+          if (is_3d(F)) r2.tighten_lb(r3.lb);
+          IV r1 = r1_;
+          r1.tighten_lb(r2.lb);
+          for (int kind = 1; kind <= 3; ++kind) {
+            if (!(kind == 3 ? nofit : (mask >> (kind - 1)) & 1)) continue;
+            const unsigned long long slot = atomicAdd(P.misc + MR_NREC, 1ull);
+            if (slot < P.crit_cap) {
+              double* rec = P.crit_out + 9 * (size_t)slot;
+              rec[0] = (double)kind;
+              rec[1] = la.lb; rec[2] = la.ub; rec[3] = r1.lb; rec[4] = r1.ub;
+              rec[5] = r2.lb; rec[6] = r2.ub; rec[7] = r3.lb; rec[8] = r3.ub;
+            }
           }
         }
-      } else if (v == V_NOFIT) {
-        atomicOr(P.status + 1, 1u);
-      } else if (v == V_TRAP) {
-        atomicCAS(P.status, 0u, (uint32_t)(vb >> 4));
       }
     }
+    if (__ballot(best < published)) {  // publish the wave's new minimum at once
+      published = wave_min_u64(best);
+      if (lane == 0) atomicMin(first, published);
+    }
   }
-  // wave min / sum, one atomic each
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(best, off, 64);
-    best = o < best ? o : best;
-    ev += __shfl_xor(ev, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (best != ~0ull) atomicMin(P.first, best);
-    if (ev) atomicAdd(P.evaluated + 16 * (((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & 63), ev);
+  ev = wave_sum_u64(ev);
+  nofit_flags = wave_sum_u64(nofit_flags);
+  if (lane == 0) {
+    unsigned long long* slot = P.misc + MR_SLOTS + MR_SLOT_STRIDE * (wave & (MR_NSLOTS - 1));
+    if (ev) atomicAdd(slot + 0, ev);
+    if (skipped) atomicAdd(slot + 1, skipped);
+    if (nofit_flags) atomicAdd(slot + 2, nofit_flags);
   }
 }
 
@@ -1191,8 +1272,9 @@ template <int F> struct LaunchRecord {
   }
 };
 template <int F> struct LaunchModeR {
-  static void run(unsigned grid, const ModeRParams& P) {
-    hipLaunchKernelGGL(k_mode_r<F>, dim3(grid), dim3(256), 0, g.stream, P);
+  static void run(bool out, unsigned grid, const ModeRParams& P) {
+    if (out) hipLaunchKernelGGL((k_mode_r<F, true>), dim3(grid), dim3(256), 0, g.stream, P);
+    else hipLaunchKernelGGL((k_mode_r<F, false>), dim3(grid), dim3(256), 0, g.stream, P);
   }
 };
 
@@ -2116,99 +2198,143 @@ static const char* kCritText[CPL_NUM_FAMILIES][3] = {
      "End configuration (3 disks): Critical interval with r1 on the outside - "},
 };
 
-int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, int output_criticals,
-                          double* first_problematic_out, uint64_t* cells_evaluated_out, char* out_text,
-                          size_t out_text_cap) {
-  int rc = ensure_init();
-  if (rc) return rc;
-  if (family < 0 || family >= CPL_NUM_FAMILIES || !first_problematic_out)
-    return fail(CPL_ERR_ARG, "cpl_first_problematic: bad argument");
-  int32_t n[4];
+namespace {
+
+// Subdivision counts of a launch (reference counts when subdiv is NULL).
+int mode_r_subdiv(int family, const int32_t* subdiv, int32_t n[4]) {
+  if (family < 0 || family >= CPL_NUM_FAMILIES) return fail(CPL_ERR_ARG, "cpl_first_problematic: bad family");
   for (int k = 0; k < 4; ++k) n[k] = subdiv ? subdiv[k] : kRefSubdiv[family][k];
   if (!(family == 0 || family == 2 || family == 3 || family == 5)) n[3] = 1;
   for (int k = 0; k < 4; ++k)
     if (n[k] < 1) return fail(CPL_ERR_ARG, "subdivision counts must be >= 1");
-  if ((rc = g.mr_ranges.ensure((size_t)n[0] * 4 * sizeof(double))) || (rc = g.mr_misc.ensure(256 + 64 * 128)))
-    return rc;
-  const uint32_t crit_cap = output_criticals ? (1u << 20) : 0u;
-  if (crit_cap && (rc = g.mr_crit.ensure((size_t)crit_cap * 9 * sizeof(double)))) return rc;
+  return CPL_OK;
+}
 
+// Enqueues one reference-grid launch on the library stream: clear the scratch,
+// per-lambda r1 ranges, the family's kernel.  ranges: n[0] x 4 doubles; misc:
+// MODE_R_MISC_BYTES; crit: crit_cap records of 9 doubles (output mode).
+int mode_r_enqueue(int family, cpl_iv outer_la, const int32_t n[4], bool out, bool exhaustive, double* ranges,
+                   unsigned long long* misc, double* crit, uint32_t crit_cap) {
   ModeRParams P;
   P.outer = IV(outer_la.lb, outer_la.ub);
   for (int k = 0; k < 4; ++k) P.n[k] = n[k];
-  P.r1_ranges = g.mr_ranges.as<double>();
-  char* misc = g.mr_misc.as<char>();
-  P.first = reinterpret_cast<unsigned long long*>(misc);
-  P.evaluated = reinterpret_cast<unsigned long long*>(misc + 256);  // 64 slots, one 128-byte line each
-  P.status = reinterpret_cast<uint32_t*>(misc + 16);
-  P.crit_count = reinterpret_cast<uint32_t*>(misc + 24);
-  P.output_criticals = output_criticals;
-  P.crit_out = crit_cap ? g.mr_crit.as<double>() : nullptr;
-  P.crit_cap = crit_cap;
-
-  // h_first_problematic.set(DBL_MAX) + copy (run_kernel.cuh:81-82)
-  unsigned long long init[4];
-  {
-    double dm = DBL_MAX;
-    uint64_t b;
-    std::memcpy(&b, &dm, 8);
-    init[0] = b ^ ((0 - (b >> 63)) | (1ull << 63));
-    init[1] = 0;
-    init[2] = 0;
-    init[3] = 0;
-  }
-  HIP_TRY(hipMemcpyAsync(misc, init, sizeof(init), hipMemcpyHostToDevice, g.stream));
-  HIP_TRY(hipMemsetAsync(misc + 256, 0, 64 * 128, g.stream));
+  P.r1_ranges = ranges;
+  P.misc = misc;
+  P.exhaustive = exhaustive ? 1 : 0;
+  P.crit_out = crit;
+  P.crit_cap = crit ? crit_cap : 0u;
+  hipLaunchKernelGGL(k_mode_r_init, dim3(1), dim3(256), 0, g.stream, misc);
   hipLaunchKernelGGL(k_mode_r_ranges, dim3((unsigned)((n[0] + 255) / 256)), dim3(256), 0, g.stream, P.outer, n[0],
-                     g.mr_ranges.as<double>());
-  HIP_TRY(hipGetLastError());
-  // one block per (la, r1) row, grid stride over rows
+                     ranges);
+  // 16 blocks of four waves per CU; waves take (la, r1) rows in turn
   const uint64_t rows = (uint64_t)n[0] * n[1];
-  dispatch_family<LaunchModeR>(family, (unsigned)std::min<uint64_t>(rows, (uint64_t)std::max(g.cu_count, 1) * 16), P);
+  const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((rows + 3) / 4, (uint64_t)std::max(g.cu_count, 1) * 16));
+  dispatch_family<LaunchModeR>(family, out, (unsigned)blocks, P);
   HIP_TRY(hipGetLastError());
-  unsigned long long back[4];
-  std::vector<unsigned long long> evs(64 * 16);
-  HIP_TRY(hipMemcpyAsync(back, misc, sizeof(back), hipMemcpyDeviceToHost, g.stream));
-  HIP_TRY(hipMemcpyAsync(evs.data(), misc + 256, evs.size() * 8, hipMemcpyDeviceToHost, g.stream));
-  HIP_TRY(hipStreamSynchronize(g.stream));
-  back[1] = 0;
-  for (int k = 0; k < 64; ++k) back[1] += evs[16 * k];
+  return CPL_OK;
+}
+
+// Reads a launch's scratch back (after the stream has been synchronised) and
+// applies the failure rule: a trap or fit-check failure counts where its la.lb
+// is below the first problematic la.lb (every cell there is reached by the
+// reference), or anywhere in output mode.
+int mode_r_finish(bool out, const unsigned long long* misc, double* first_out, cpl_mode_r_info* info) {
+  const uint64_t v = misc[MR_FIRST];
   {
-    uint64_t v = back[0];
     uint64_t mask = ((v >> 63) - 1) | (1ull << 63);
     uint64_t b = v ^ mask;
-    std::memcpy(first_problematic_out, &b, 8);
+    std::memcpy(first_out, &b, 8);
   }
-  if (cells_evaluated_out) *cells_evaluated_out = back[1];
-  const uint32_t trap_code = (uint32_t)(back[2] & 0xFFFFFFFFu), nofit = (uint32_t)(back[2] >> 32);
-  const uint32_t n_crit = (uint32_t)(back[3] & 0xFFFFFFFFu);
-  if (output_criticals && out_text && out_text_cap) {
-    std::vector<double> recs((size_t)std::min(n_crit, crit_cap) * 9);
-    if (!recs.empty())
-      HIP_TRY(hipMemcpy(recs.data(), g.mr_crit.p, recs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    size_t used = 0;
-    out_text[0] = 0;
-    for (size_t k = 0; k < recs.size() / 9; ++k) {
-      const double* r = &recs[9 * k];
-      int which = (int)r[0];
-      if (which < 1 || which > 2) which = 1;
-      char line[512];
-      int len;
-      if (family == 1 || family == 4)
-        len = snprintf(line, sizeof line, "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g]\n",
-                       kCritText[family][which], r[1], r[2], r[3], r[4], r[5], r[6]);
-      else
-        len = snprintf(line, sizeof line,
-                       "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g], r3 = [%.19g,%.19g]\n",
-                       kCritText[family][which], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]);
-      if (len < 0 || used + (size_t)len + 1 > out_text_cap) break;
+  cpl_mode_r_info I;
+  std::memset(&I, 0, sizeof I);
+  for (int k = 0; k < MR_NSLOTS; ++k) {
+    const unsigned long long* slot = misc + MR_SLOTS + MR_SLOT_STRIDE * k;
+    I.cells_evaluated += slot[0];
+    I.rows_skipped += slot[1];
+    I.nofit_flags += slot[2];
+  }
+  I.lines_found = out ? misc[MR_NREC] : 0;
+  const uint64_t limit = out ? ~0ull : v;
+  int code = 0;
+  for (int c = 1; c < 6; ++c)
+    if (misc[MR_FAIL + c] < limit && (code == 0 || misc[MR_FAIL + c] < misc[MR_FAIL + code])) code = c;
+  I.trap_code = code;
+  I.fit_check_failed = misc[MR_FAIL] < limit ? 1 : 0;
+  if (info) *info = I;
+  if (code) return fail(CPL_ERR_TRAP, "a cell hit a trap condition (code " + std::to_string(code) + ")");
+  if (I.fit_check_failed) return fail(CPL_ERR_NOFIT, "circle possibly does not fit (middle/end fit check)");
+  return CPL_OK;
+}
+
+// The printed line of a record, in the reference's printf formats
+// (start_configuration.cu:132,147,158,207,289; middle_configuration.cu:182,193;
+// end_configuration.cu:124,135,245,256).
+int format_record(int family, const double* r, char* line, size_t cap) {
+  const int kind = (int)r[0];
+  const char* head = kind == 3 ? "Start configuration: Circle possibly does not fit - "
+                               : kCritText[family][kind == 2 ? 2 : 1];
+  if (family == 1 || family == 4)
+    return snprintf(line, cap, "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g]\n", head, r[1], r[2],
+                    r[3], r[4], r[5], r[6]);
+  return snprintf(line, cap, "%sla = [%.19g,%.19g], r1 = [%.19g,%.19g], r2 = [%.19g,%.19g], r3 = [%.19g,%.19g]\n",
+                  head, r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]);
+}
+
+// Appends the lines of `have` records to out_text (as many as fit) and
+// completes the report's line counts and text size.
+void write_lines(int family, const double* recs, uint64_t have, char* out_text, size_t out_text_cap,
+                 cpl_mode_r_info& I) {
+  size_t used = 0, needed = 1;
+  if (out_text && out_text_cap) out_text[0] = 0;
+  for (uint64_t k = 0; k < have; ++k) {
+    char line[512];
+    const int len = format_record(family, recs + 9 * k, line, sizeof line);
+    if (len < 0) continue;
+    needed += (size_t)len;
+    if (out_text && used + (size_t)len + 1 <= out_text_cap) {
       std::memcpy(out_text + used, line, (size_t)len + 1);
       used += (size_t)len;
+      ++I.lines_written;
     }
   }
-  if (trap_code) return fail(CPL_ERR_TRAP, "a cell hit a trap condition (code " + std::to_string(trap_code) + ")");
-  if (nofit) return fail(CPL_ERR_NOFIT, "circle possibly does not fit (middle/end fit check)");
-  return CPL_OK;
+  I.text_bytes_needed = needed;
+}
+
+}  // namespace
+
+int cpl_first_problematic_ex(int family, cpl_iv outer_la, const int32_t* subdiv, uint32_t flags,
+                             double* first_problematic_out, cpl_mode_r_info* info, char* out_text,
+                             size_t out_text_cap) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!first_problematic_out) return fail(CPL_ERR_ARG, "cpl_first_problematic: null output");
+  if (flags & ~(uint32_t)(CPL_MR_OUTPUT_CRITICALS | CPL_MR_EXHAUSTIVE))
+    return fail(CPL_ERR_ARG, "cpl_first_problematic_ex: unknown flag");
+  int32_t n[4];
+  if ((rc = mode_r_subdiv(family, subdiv, n))) return rc;
+  const bool out = (flags & CPL_MR_OUTPUT_CRITICALS) != 0;
+  const uint32_t crit_cap = out ? CPL_MR_MAX_RECORDS : 0u;
+  if ((rc = g.mr_ranges.ensure((size_t)n[0] * 4 * sizeof(double))) || (rc = g.mr_misc.ensure(MODE_R_MISC_BYTES)))
+    return rc;
+  if (crit_cap && (rc = g.mr_crit.ensure((size_t)crit_cap * 9 * sizeof(double)))) return rc;
+  if ((rc = mode_r_enqueue(family, outer_la, n, out, (flags & CPL_MR_EXHAUSTIVE) != 0, g.mr_ranges.as<double>(),
+                           g.mr_misc.as<unsigned long long>(), crit_cap ? g.mr_crit.as<double>() : nullptr, crit_cap)))
+    return rc;
+  std::vector<unsigned long long> misc(MODE_R_MISC_BYTES / 8);
+  HIP_TRY(hipMemcpyAsync(misc.data(), g.mr_misc.p, MODE_R_MISC_BYTES, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  cpl_mode_r_info I;
+  const int result = mode_r_finish(out, misc.data(), first_problematic_out, &I);
+  const std::string err = g_err;
+  if (out) {
+    const uint64_t have = std::min<uint64_t>(I.lines_found, crit_cap);
+    std::vector<double> recs((size_t)have * 9);
+    if (have) HIP_TRY(hipMemcpy(recs.data(), g.mr_crit.p, recs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    write_lines(family, recs.data(), have, out_text, out_text_cap, I);
+  }
+  if (info) *info = I;
+  g_err = err;
+  return result;
 }
 
 }  // extern "C"

@@ -682,6 +682,76 @@ __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3
   return vbyte(V_PROVEN, nofit, 0);
 }
 
+// The reference's output_criticals mode on one cell: the kernels keep
+// evaluating after a failing criterion and print one line per failure
+// (start_configuration.cu:142-161, :202-210, :284-292; middle_configuration.cu:
+// 180-200; end_configuration.cu:122-142, :243-263).  Returns the mask of
+// failing criteria (bit k-1: criterion k), or 16 + trap code when a check of
+// any criterion traps; *nofit_flag: the start family's "possibly does not fit"
+// line (start_configuration.cu:131-135, printed before the criteria run);
+// *early: V_EMPTY, or V_NOFIT for the families that trap at the fit check,
+// when the cell is decided before the criteria (mask 0), else -1.
+template <int F>
+__device__ __forceinline__ int eval_kernel_body_all(IV la, IV r1_, IV r2_, IV r3, int* nofit_flag, int* early) {
+  IV r2 = r2_;
+  if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
+  IV r1 = r1_;
+  r1.tighten_lb(r2.lb);
+  *nofit_flag = 0;
+  *early = -1;
+  if (r1.empty() || r2.empty()) {
+    *early = V_EMPTY;
+    return 0;
+  }
+  if constexpr (F == F_START || F == F_MIDDLE || F == F_END_2D || F == F_END_3D) {
+    if (mul_ru(2.0, r1.ub) >= add_rd(1.0, -la.ub)) {
+      if constexpr (F != F_START) {
+        *early = V_NOFIT;
+        return 0;
+      }
+      *nofit_flag = 1;
+    }
+  }
+  int trap = 0, mask = 0;
+  if constexpr (F == F_START) {
+    IV a12 = collision_angle_outer_inner(la, r1, r2, trap);
+    if (trap) return 16 + trap;
+    IV a13 = collision_angle_outer_outer(r1, r3, trap);
+    if (trap) return 16 + trap;
+    IV a23 = collision_angle_inner_outer(la, r2, r3, trap);
+    if (trap) return 16 + trap;
+    UB case1 = (a12 + a23 > a13);
+    if (possibly(case1) && !handle_case_1<double>(la, r1, r2, a12, nullptr)) mask |= 1;
+    if (possibly(!case1) && !handle_case_2<double>(la, r1, r2, r3, a13, nullptr)) mask |= 2;
+  } else if constexpr (F == F_START_END_2D) {
+    const bool ok = handle_2d_case<double>(la, r1, r2, trap, nullptr);
+    if (trap) return 16 + trap;
+    mask = ok ? 0 : 1;
+  } else if constexpr (F == F_START_END_3D) {
+    const bool ok = handle_3d_case<double>(la, r1, r2, r3, trap, nullptr);
+    if (trap) return 16 + trap;
+    mask = ok ? 0 : 1;
+  } else {
+    bool ok1, ok2;
+    if constexpr (F == F_MIDDLE) {
+      ok1 = handle_case_1_1<double>(la, r1, r2, r3, trap);
+      if (trap) return 16 + trap;
+      ok2 = handle_case_2_1<double>(la, r1, r2, r3, trap);
+    } else if constexpr (F == F_END_2D) {
+      ok1 = handle_inner_outer_2d<double>(la, r1, r2, trap);
+      if (trap) return 16 + trap;
+      ok2 = handle_outer_inner_2d<double>(la, r1, r2, trap, nullptr);
+    } else {
+      ok1 = handle_r1_inner_3d<double>(la, r1, r2, r3, trap);
+      if (trap) return 16 + trap;
+      ok2 = handle_r1_outer_3d<double>(la, r1, r2, r3, trap);
+    }
+    if (trap) return 16 + trap;
+    mask = (ok1 ? 0 : 1) | (ok2 ? 0 : 2);
+  }
+  return mask;
+}
+
 // r3 from a position interval p in [0,1] over the admissible r3 range; for a
 // root cell p = [k/n, (k+1)/n] this reproduces get_subinterval(range, k, n).
 __device__ __forceinline__ IV r3_from_position(IV range, IV p) {

@@ -13,7 +13,10 @@
 //   --checkpoint FILE  after every launch, save the stage and its lambda queue
 //   --resume FILE      continue a run from a saved queue (earlier stages are
 //                      skipped; the saved stage resumes with its queue)
+//   --stage-times      after each stage, print its launch count, wall time and
+//                      cells evaluated to stderr (no counterpart in the reference)
 #include <cfloat>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -61,6 +64,7 @@ struct Options {
   bool resume = false;
   int resume_family = -1;
   std::vector<circlepacking::IV> resume_queue;
+  bool stage_times = false;
 };
 Options g_opt;
 
@@ -69,16 +73,27 @@ struct StageLimit {};
 struct HipLauncher {
   int family;
   long launches = 0;
+  uint64_t cells = 0, rows_skipped = 0;
   double operator()(circlepacking::IV la, bool output_criticals) {
     if (g_opt.max_launches >= 0 && launches >= g_opt.max_launches) throw StageLimit{};
     ++launches;
     double first = DBL_MAX;
-    uint64_t evaluated = 0;
-    std::vector<char> text(output_criticals ? (64u << 20) : 0);
+    cpl_mode_r_info info;
+    // every line kept on the device fits (a line of the reference's formats is < 384 bytes)
+    std::vector<char> text(output_criticals ? (size_t)CPL_MR_MAX_RECORDS * 384 : 0);
     cpl_iv iv{la.lb, la.ub};
-    int rc = cpl_first_problematic(family, iv, g_opt.have_subdiv ? g_opt.subdiv : nullptr, output_criticals ? 1 : 0,
-                                   &first, &evaluated, output_criticals ? text.data() : nullptr, text.size());
-    if (output_criticals && !text.empty()) std::fputs(text.data(), stdout);
+    int rc = cpl_first_problematic_ex(family, iv, g_opt.have_subdiv ? g_opt.subdiv : nullptr,
+                                      output_criticals ? CPL_MR_OUTPUT_CRITICALS : 0u, &first, &info,
+                                      output_criticals ? text.data() : nullptr, text.size());
+    cells += info.cells_evaluated;
+    rows_skipped += info.rows_skipped;
+    if (output_criticals && !text.empty()) {
+      std::fputs(text.data(), stdout);
+      std::fflush(stdout);
+      if (info.lines_written < info.lines_found)
+        std::cerr << "Critical-cell output truncated: " << info.lines_written << " of " << info.lines_found
+                  << " lines shown" << std::endl;
+    }
     throw_if_cpl_error(rc);
     return first;
   }
@@ -97,7 +112,15 @@ bool run_stage(int family) {
     cp.resume = true;
     cp.resume_from = g_opt.resume_queue;
   }
-  return circlepacking::run_kernel_until_no_criticals(launch, circlepacking::IV(g_opt.lo, g_opt.hi), std::cerr, &cp);
+  const auto t0 = std::chrono::steady_clock::now();
+  const bool done =
+      circlepacking::run_kernel_until_no_criticals(launch, circlepacking::IV(g_opt.lo, g_opt.hi), std::cerr, &cp);
+  if (g_opt.stage_times) {
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::cerr << "Stage " << family << ": " << launch.launches << " launches, " << sec << " s, " << launch.cells
+              << " cells evaluated, " << launch.rows_skipped << " rows skipped" << std::endl;
+  }
+  return done;
 }
 
 // Stage selection: --family, and on resume every stage before the saved one is done.
@@ -173,6 +196,8 @@ int main(int argc, char** argv) {
       if (std::sscanf(next(), "%lf,%lf", &g_opt.lo, &g_opt.hi) != 2) return 2;
     } else if (a == "--max-launches") {
       g_opt.max_launches = std::atol(next());
+    } else if (a == "--stage-times") {
+      g_opt.stage_times = true;
     } else if (a == "--checkpoint") {
       g_opt.checkpoint = next();
     } else if (a == "--resume") {
@@ -187,7 +212,7 @@ int main(int argc, char** argv) {
     } else {
       std::fprintf(stderr,
                    "usage: %s [--family F] [--subdiv a,b,c,d] [--lambda lo,hi] [--max-launches N] "
-                   "[--checkpoint FILE] [--resume FILE]\n",
+                   "[--checkpoint FILE] [--resume FILE] [--stage-times]\n",
                    argv[0]);
       return 2;
     }
@@ -199,6 +224,11 @@ int main(int argc, char** argv) {
     std::cerr << "Launch limit reached" << std::endl;
     cpl_shutdown();
     return 3;
+  } catch (const std::system_error& e) {
+    // the reference's kernels trap here and its next synchronising copy throws
+    std::cerr << "error: " << e.code().message() << ": " << e.what() << std::endl;
+    cpl_shutdown();
+    return 1;
   }
   cpl_shutdown();
   return 0;  // prover.cpp:35: the result of each stage is not inspected

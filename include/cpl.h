@@ -162,13 +162,56 @@ const char* cpl_last_error(void);
  * end_configuration.cu:39-53), as run_kernel.cuh:81-84 does it: writes the
  * smallest la.lb of a cell failing a criterion, or DBL_MAX if none fails.
  * subdiv: NULL for the reference counts, else 4 counts la, r1, r2, r3 (r3
- * ignored by 2-d families) for truncated runs.  With output_criticals, every
- * critical cell is appended to the text buffer in the reference's %.19g
- * format (the reference printf lines); out_text may be NULL.
- * Returns CPL_ERR_TRAP / CPL_ERR_NOFIT where the reference would trap. */
+ * ignored by 2-d families) for truncated runs.  The same launch as
+ * cpl_first_problematic_ex with flags = output_criticals ? CPL_MR_OUTPUT_CRITICALS
+ * : 0 (see there: early exit, failure rule, printed lines); cells_evaluated_out
+ * and out_text may be NULL.  Returns CPL_ERR_TRAP / CPL_ERR_NOFIT where the
+ * reference would trap. */
 int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, int output_criticals,
                           double* first_problematic_out, uint64_t* cells_evaluated_out,
                           char* out_text, size_t out_text_cap);
+
+/* Options and report of cpl_first_problematic_ex. */
+enum {
+  CPL_MR_OUTPUT_CRITICALS = 1, /* the reference's output_criticals = true (run_kernel.cuh:100-103) */
+  CPL_MR_EXHAUSTIVE = 2        /* evaluate every cell (no early exit; cells_evaluated is the grid's count) */
+};
+#define CPL_MR_MAX_RECORDS (1u << 20) /* lines kept on the device per output-mode launch */
+typedef struct {
+  uint64_t cells_evaluated;   /* cells whose kernel body ran */
+  uint64_t rows_skipped;      /* (la, r1) rows the early exit skipped */
+  uint64_t nofit_flags;       /* start family: evaluated cells whose fit check fired (a printed line each) */
+  uint64_t lines_found;       /* output mode: printed lines found (failing criteria + start no-fit lines) */
+  uint64_t lines_written;     /* lines written to out_text (fewer than lines_found: truncated) */
+  uint64_t text_bytes_needed; /* bytes the text of the lines kept on the device needs, with the final NUL */
+  int32_t trap_code;          /* trap reported (CPL_TRAP_*; 0 none) */
+  int32_t fit_check_failed;   /* the middle/end fit check was reported */
+} cpl_mode_r_info;
+
+/* One launch of family f over outer_la on the reference grid (the call of
+ * run_kernel.cuh:81-84 and :100-103), with a report.
+ *
+ * Without CPL_MR_OUTPUT_CRITICALS the launch mirrors the reference kernels'
+ * store_min + return (start_configuration.cu:142-145 and alike): every
+ * reference thread walks its cells in increasing la and returns at its first
+ * failing cell, so cells whose la.lb is at or above a failing la.lb found so
+ * far cannot change the result and are skipped (rows_skipped; unless
+ * CPL_MR_EXHAUSTIVE).  *first_problematic_out is the smallest la.lb of a cell
+ * failing a criterion, DBL_MAX if none, either way.  A trap or a middle/end fit
+ * failure is reported (CPL_ERR_TRAP, CPL_ERR_NOFIT) when its cell has la.lb
+ * below that minimum: those cells are reached by the reference in every thread
+ * mapping; above it the reference's outcome depends on its thread mapping.
+ *
+ * With CPL_MR_OUTPUT_CRITICALS every cell and every criterion is evaluated,
+ * every trap and fit failure counts, and each line the reference would print
+ * (one per failing criterion, and the start family's "possibly does not fit"
+ * line) is appended to out_text in the reference's %.19g formats, in no
+ * particular order.  At most CPL_MR_MAX_RECORDS lines are kept; lines_found
+ * and lines_written show truncation, text_bytes_needed the buffer size the
+ * kept lines need.  info may be NULL. */
+int cpl_first_problematic_ex(int family, cpl_iv outer_la, const int32_t* subdiv, uint32_t flags,
+                             double* first_problematic_out, cpl_mode_r_info* info, char* out_text,
+                             size_t out_text_cap);
 
 /* ---- per-cell evaluation (criterion interface) -------------------------- */
 /* Verdicts of family f for n cells given as 8 host arrays of n doubles

@@ -40,6 +40,7 @@ def test_struct_layouts_match_header(tmp_path):
         "cpl_refine_stats": (abi.RefineStats, [f[0] for f in abi.RefineStats._fields_]),
         "cpl_tune_point": (abi.TunePoint, ["block", "cells_per_thread", "seconds"]),
         "cpl_iv": (abi.Iv, ["lb", "ub"]),
+        "cpl_mode_r_info": (abi.ModeRInfo, [f[0] for f in abi.ModeRInfo._fields_]),
     }
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "cpl.h"', "int main(void) {"]
     for ctype, (_, names) in fields.items():

@@ -64,49 +64,3 @@ def test_4param_full_grid_root_sample(cpl, cplo):
     assert np.array_equal(gf[::stride], of)
     assert g.certified + g.unresolved == sum(
         g.evaluations[d] - (g.evaluations[d + 1] // 2 if d + 1 < g.rounds else 0) for d in range(g.rounds))
-
-
-@pytest.mark.parametrize("family", range(6))
-def test_reference_grid_odd_subdivisions(cpl, cplo, family):
-    """Rows (la, r1) longer than a block, cell counts per row that 256 does not
-    divide, and r3 counts below and above 256: every cell of the grid is
-    evaluated once (same count and same first problematic la as the oracle)."""
-    shapes = [(5, 7, 9, 300), (6, 7, 300, 3), (3, 5, 11, 37)] if abi.IS_3D[family] else [(7, 9, 1000, 1), (5, 3, 257, 1)]
-    for subdiv in shapes:
-        for outer in [(abi.LAMBDA_MIN, abi.LAMBDA_MAX), (0.55, 0.6)]:
-            rc, first, ev, trap = cplo.first_problematic(family, outer, subdiv)
-            if rc == 0:
-                gfirst, gev, _ = cpl.first_problematic(family, outer, subdiv)
-                assert (gfirst, gev) == (first, ev), (subdiv, outer)
-            else:
-                with pytest.raises(cpl.CplError) as ei:
-                    cpl.first_problematic(family, outer, subdiv)
-                assert ei.value.code == rc
-
-
-# One launch per family over [lambda_min, lambda_max] on the reference's full
-# grids (1e9 to 7e10 cells): first problematic la.lb and cells evaluated, as
-# first measured with one flat cell index per thread (too large for the oracle;
-# the truncated and odd-subdivision grids above are its parity checks).
-FULL_GRID = {0: (0.5899609375, 1073741824), 1: (0.777060546875, 68719476736), 2: (0.5937890625, 1073741824),
-             3: (0.6990625, 2147483648), 4: (0.7768212890625, 4294967296), 5: (0.8474023437500001, 2167963008)}
-
-
-@pytest.mark.parametrize("family", range(6))
-def test_reference_grid_full_first_launch(cpl, family):
-    first, ev, _ = cpl.first_problematic(family, (abi.LAMBDA_MIN, abi.LAMBDA_MAX))
-    assert (first, ev) == FULL_GRID[family]
-
-
-@pytest.mark.parametrize("family", range(6))
-def test_reference_grid_truncated(cpl, cplo, family):
-    subdiv = (8, 8, 8, 4) if abi.IS_3D[family] else (16, 16, 16, 1)
-    for outer in [(abi.LAMBDA_MIN, abi.LAMBDA_MAX), (0.5, 0.6), (0.7, 0.71)]:
-        rc, first, ev, trap = cplo.first_problematic(family, outer, subdiv)
-        if rc == 0:
-            gfirst, gev, _ = cpl.first_problematic(family, outer, subdiv)
-            assert (gfirst, gev) == (first, ev)
-        else:
-            with pytest.raises(cpl.CplError) as ei:
-                cpl.first_problematic(family, outer, subdiv)
-            assert ei.value.code == rc
