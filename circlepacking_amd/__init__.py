@@ -55,6 +55,7 @@ def lib():
                                             _P_D, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
         L.cpl_first_problematic_ex.argtypes = [ctypes.c_int, abi.Iv, ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32,
                                                _P_D, ctypes.POINTER(abi.ModeRInfo), ctypes.c_char_p, ctypes.c_size_t]
+        L.cpl_first_problematic_noalloc.argtypes = L.cpl_first_problematic_ex.argtypes
         _lib = L
     return _lib
 
@@ -172,10 +173,12 @@ def first_problematic(family, outer_la, subdiv=None, output_criticals=False, tex
     return first.value, ev.value, (buf.value.decode() if buf is not None else "")
 
 
-def first_problematic_ex(family, outer_la, subdiv=None, output_criticals=False, exhaustive=False, text_cap=1 << 22):
+def first_problematic_ex(family, outer_la, subdiv=None, output_criticals=False, exhaustive=False, text_cap=1 << 22,
+                         noalloc=False):
     """One reference-grid launch with its report: (first problematic la.lb or
     DBL_MAX, info dict, printed lines as text).  Raises CplError(-3/-4) where
-    the reference would trap; the error carries .first, .info and .text."""
+    the reference would trap; the error carries .first, .info and .text.
+    noalloc: through cpl_first_problematic_noalloc (fixed device scratch)."""
     first = ctypes.c_double()
     info = abi.ModeRInfo()
     sd = None
@@ -183,7 +186,8 @@ def first_problematic_ex(family, outer_la, subdiv=None, output_criticals=False, 
         sd = (ctypes.c_int32 * 4)(*subdiv)
     flags = (abi.MR_OUTPUT_CRITICALS if output_criticals else 0) | (abi.MR_EXHAUSTIVE if exhaustive else 0)
     buf = ctypes.create_string_buffer(text_cap) if output_criticals else None
-    rc = lib().cpl_first_problematic_ex(family, abi.Iv(*outer_la), sd, flags, ctypes.byref(first), ctypes.byref(info), buf,
+    fn = lib().cpl_first_problematic_noalloc if noalloc else lib().cpl_first_problematic_ex
+    rc = fn(family, abi.Iv(*outer_la), sd, flags, ctypes.byref(first), ctypes.byref(info), buf,
             text_cap if buf is not None else 0)
     text = buf.value.decode() if buf is not None else ""
     if rc != 0:

@@ -75,6 +75,7 @@ class Iv(ctypes.Structure):
 
 MR_OUTPUT_CRITICALS, MR_EXHAUSTIVE = 1, 2
 MR_MAX_RECORDS = 1 << 20
+NOALLOC_MAX_LA, NOALLOC_MAX_RECORDS = 4096, 4096
 
 
 class ModeRInfo(ctypes.Structure):

@@ -1137,6 +1137,18 @@ __global__ __launch_bounds__(256, 4) void k_mode_r(ModeRParams P) {
   }
 }
 
+// Device scratch of cpl_first_problematic_noalloc: the per-lambda r1 ranges
+// of up to CPL_NOALLOC_MAX_LA lambda subdivisions, one launch's MODE_R_MISC_BYTES
+// and up to CPL_NOALLOC_MAX_RECORDS output-mode records.  Every GPU the library
+// runs on holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns at initialisation.
+constexpr size_t kNoAllocRangesBytes = (size_t)CPL_NOALLOC_MAX_LA * 4 * sizeof(double);
+constexpr size_t kNoAllocRecordBytes = (size_t)CPL_NOALLOC_MAX_RECORDS * 9 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocRangesBytes + MODE_R_MISC_BYTES + kNoAllocRecordBytes;
+static_assert(kNoAllocRangesBytes % 256 == 0 && MODE_R_MISC_BYTES % 256 == 0, "scratch sections stay aligned");
+static_assert(kNoAllocBytes == CPL_NOALLOC_SCRATCH_BYTES, "the scratch size cpl.h states");
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 // ============================================================================
 // Host side: context and C ABI
 // ============================================================================
@@ -1150,6 +1162,7 @@ struct Context {
   bool ready = false;
   int device = 0;
   int cu_count = 0;
+  unsigned char* noalloc_scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
   hipStream_t stream = nullptr;
   hipStream_t fam_stream[CPL_NUM_FAMILIES] = {};   // per-family branches (default priority)
   hipEvent_t fork_ev = nullptr, join_ev[CPL_NUM_FAMILIES] = {};
@@ -1183,6 +1196,11 @@ int ensure_init() {
     HIP_TRY(hipEventCreateWithFlags(&g.join_ev[f], hipEventDisableTiming));
   }
   HIP_TRY(hipEventCreateWithFlags(&g.fork_ev, hipEventDisableTiming));
+  // the device address of this device's copy of the no-allocation scratch
+  void* sp = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  if (!sp) return fail(CPL_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+  g.noalloc_scratch = static_cast<unsigned char*>(sp);
   g.ready = true;
   return CPL_OK;
 }
@@ -2335,6 +2353,66 @@ int cpl_first_problematic_ex(int family, cpl_iv outer_la, const int32_t* subdiv,
   if (info) *info = I;
   g_err = err;
   return result;
+}
+
+// Host staging of the no-allocation entry point (static: the entry point
+// allocates nothing; calls are not reentrant, as every entry point here).
+static unsigned long long g_noalloc_misc[MODE_R_MISC_BYTES / 8];
+static double g_noalloc_recs[(size_t)CPL_NOALLOC_MAX_RECORDS * 9];
+
+int cpl_first_problematic_noalloc(int family, cpl_iv outer_la, const int32_t* subdiv, uint32_t flags,
+                                  double* first_problematic_out, cpl_mode_r_info* info, char* out_text,
+                                  size_t out_text_cap) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != g.device)
+    return fail(CPL_ERR_ARG, "cpl_first_problematic_noalloc: the library context belongs to device " +
+                                 std::to_string(g.device) + ", the current device is " + std::to_string(dev));
+  if (!first_problematic_out) return fail(CPL_ERR_ARG, "cpl_first_problematic_noalloc: null output");
+  if (flags & ~(uint32_t)(CPL_MR_OUTPUT_CRITICALS | CPL_MR_EXHAUSTIVE))
+    return fail(CPL_ERR_ARG, "cpl_first_problematic_noalloc: unknown flag");
+  int32_t n[4];
+  if ((rc = mode_r_subdiv(family, subdiv, n))) return rc;
+  if (n[0] > (int32_t)CPL_NOALLOC_MAX_LA)
+    return fail(CPL_ERR_ARG, "cpl_first_problematic_noalloc: " + std::to_string(n[0]) +
+                                 " lambda subdivisions exceed the scratch's maximum of " +
+                                 std::to_string(CPL_NOALLOC_MAX_LA));
+  const bool out = (flags & CPL_MR_OUTPUT_CRITICALS) != 0;
+  double* ranges = reinterpret_cast<double*>(g.noalloc_scratch);
+  unsigned long long* misc = reinterpret_cast<unsigned long long*>(g.noalloc_scratch + kNoAllocRangesBytes);
+  double* crit = reinterpret_cast<double*>(g.noalloc_scratch + kNoAllocRangesBytes + MODE_R_MISC_BYTES);
+  if ((rc = mode_r_enqueue(family, outer_la, n, out, (flags & CPL_MR_EXHAUSTIVE) != 0, ranges, misc,
+                           out ? crit : nullptr, CPL_NOALLOC_MAX_RECORDS)))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(g_noalloc_misc, misc, MODE_R_MISC_BYTES, hipMemcpyDeviceToHost, g.stream));
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  cpl_mode_r_info I;
+  const int result = mode_r_finish(out, g_noalloc_misc, first_problematic_out, &I);
+  const std::string err = g_err;
+  if (out) {
+    const uint64_t have = std::min<uint64_t>(I.lines_found, CPL_NOALLOC_MAX_RECORDS);
+    if (have) {
+      HIP_TRY(hipMemcpyAsync(g_noalloc_recs, crit, have * 9 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+      HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    write_lines(family, g_noalloc_recs, have, out_text, out_text_cap, I);
+  }
+  if (info) *info = I;
+  g_err = err;
+  return result;
+}
+
+int cpl_first_problematic(int family, cpl_iv outer_la, const int32_t* subdiv, int output_criticals,
+                          double* first_problematic_out, uint64_t* cells_evaluated_out, char* out_text,
+                          size_t out_text_cap) {
+  cpl_mode_r_info info;
+  std::memset(&info, 0, sizeof info);
+  const int rc = cpl_first_problematic_ex(family, outer_la, subdiv, output_criticals ? CPL_MR_OUTPUT_CRITICALS : 0u,
+                                          first_problematic_out, &info, out_text, out_text_cap);
+  if (cells_evaluated_out) *cells_evaluated_out = info.cells_evaluated;
+  return rc;
 }
 
 }  // extern "C"

@@ -213,6 +213,26 @@ int cpl_first_problematic_ex(int family, cpl_iv outer_la, const int32_t* subdiv,
                              double* first_problematic_out, cpl_mode_r_info* info, char* out_text,
                              size_t out_text_cap);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the launch of cpl_first_problematic_ex (same arguments,
+ * flags, results and report) on a fixed device scratch array of the library
+ * (CPL_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends) instead of
+ * allocated buffers; it calls no hipMalloc, hipFree, memset or allocating
+ * routine.  Limits: at most CPL_NOALLOC_MAX_LA lambda subdivisions (the
+ * reference grids use at most 4096; more returns CPL_ERR_ARG), and in output
+ * mode at most CPL_NOALLOC_MAX_RECORDS lines are kept (lines_found /
+ * lines_written show truncation).  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU race (the library is not reentrant);
+ * each call clears what it uses.  The library context (cpl_init) must belong to
+ * the current device. */
+#define CPL_NOALLOC_MAX_LA 4096u
+#define CPL_NOALLOC_MAX_RECORDS 4096u
+#define CPL_NOALLOC_SCRATCH_BYTES (CPL_NOALLOC_MAX_LA * 32u + 8448u + CPL_NOALLOC_MAX_RECORDS * 72u)
+int cpl_first_problematic_noalloc(int family, cpl_iv outer_la, const int32_t* subdiv, uint32_t flags,
+                                  double* first_problematic_out, cpl_mode_r_info* info, char* out_text,
+                                  size_t out_text_cap);
+
 /* ---- per-cell evaluation (criterion interface) -------------------------- */
 /* Verdicts of family f for n cells given as 8 host arrays of n doubles
  * (la.lb, la.ub, r1.lb, r1.ub, r2.lb, r2.ub, p3.lb, p3.ub).  Equivalent to
