@@ -46,7 +46,7 @@ PRIM_TRIG_OPS = {
     "dacos_rd": 36, "dacos_ru": 37,
 }
 # Atomic_ordered_double's transform and its inverse (the u64 image carried in a double's bits)
-PRIM_ORDERED_OPS = {"ordered": 38, "ordered_inv": 39}
+PRIM_ORDERED_OPS = {"ordered": 38, "ordered_inv": 39, "rcp_hw": 40, "rsq_hw": 41}
 IV_OPS = {
     "add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4, "square": 5, "abs": 6, "reciprocal": 7,
     "div_scalar": 8, "scalar_div": 9, "sin": 10, "cos": 11, "asin": 12, "acos": 13, "mul_scalar": 14, "neg": 15,

@@ -32,6 +32,26 @@ __device__ __forceinline__ double asin_poly_eval(double z) {
   return p;
 }
 
+// Round-to-nearest square root of t in [0, 1/4]: the hardware reciprocal
+// square root and the Newton sequence the compiler emits for sqrt(double)
+// (g = t y, h = y / 2, two half-steps, two residual corrections), without its
+// scaling of arguments below 2^-767 and its zero / infinity fix-up, neither of
+// which t in [2^-54, 1/4] needs (t = +-0 returns t, t < 0 NaN, as sqrt does).
+// Bit-identical to the compiler's sqrt there, which is correctly rounded.
+__device__ __forceinline__ double sqrt_quarter(double t) {
+  const double y = __builtin_amdgcn_rsq(t);
+  double g = t * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  double d = __builtin_fma(-g, g, t);
+  h = __builtin_fma(h, r, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, t);
+  g = __builtin_fma(d, h, g);
+  return t > 0.0 ? g : (t == 0.0 ? t : __builtin_nan(""));
+}
+
 // Both ranges evaluate the one polynomial (in x^2 below 1/2, in t = (1 - |x|)/2
 // above), so a wave whose lanes fall in both ranges evaluates it once.
 __device__ __forceinline__ double asin_rn(double x) {
@@ -44,7 +64,7 @@ __device__ __forceinline__ double asin_rn(double x) {
   if (small) {
     r = __builtin_fma(ax * z, p, ax);
   } else {
-    double s = __builtin_sqrt(t);
+    double s = sqrt_quarter(t);
     double c = t > 0.0 ? __builtin_fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
     double w = __builtin_fma(s * t, p, c);
     r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));
@@ -74,13 +94,20 @@ __device__ __forceinline__ double step5(double y, bool up) {
   return y;
 }
 
+// For x = +-0 both widened bounds are +0: five steps from asin(+-0) = +-0
+// cross zero and the sign clamps (x >= 0 for the lower, x <= 0 for the upper
+// bound) bring them back.  Zero arguments are common (clamped quotients of the
+// collision angles), and taking them early keeps the stepping loop of step5,
+// which a wave runs whenever one of its lanes needs it, off their path.
 __device__ __forceinline__ double dasin_rd(double x) {
+  if (x == 0.0) return 0.0;
   double y = step5(asin_rn(x), false);
   if (y < -1.5707963267948968) y = -1.5707963267948968;
   if (x >= 0.0 && y < 0.0) y = 0.0;
   return y;
 }
 __device__ __forceinline__ double dasin_ru(double x) {
+  if (x == 0.0) return 0.0;
   double y = step5(asin_rn(x), true);
   if (y > 1.5707963267948968) y = 1.5707963267948968;
   if (x <= 0.0 && y > 0.0) y = 0.0;

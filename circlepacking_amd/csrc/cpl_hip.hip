@@ -82,6 +82,8 @@ __global__ void k_debug_prim(int op, const double* a, const double* b, double* o
     case 37: r = dacos_ru(x); break;
     case 38: r = dfrom(ordered_of(x)); break;         // the u64 image carried in the bits
     case 39: r = double_of_ordered(dbits(x)); break;  // inverse of the image in x's bits
+    case 40: r = __builtin_amdgcn_rcp(x); break;      // the hardware reciprocal estimate
+    case 41: r = __builtin_amdgcn_rsq(x); break;      // the hardware reciprocal square root estimate
     default: r = __builtin_nan("");
   }
   out[i] = r;
@@ -1382,7 +1384,7 @@ int cpl_debug_fp64_peak(int iters, double* tflops) {
 int cpl_debug_prim(int op, const double* a, const double* b, double* out, size_t n) {
   int rc = ensure_init();
   if (rc) return rc;
-  if (op < 0 || op > 39 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
+  if (op < 0 || op > 41 || !a || !out) return fail(CPL_ERR_ARG, "cpl_debug_prim: bad op or pointer");
   if (n == 0) return CPL_OK;
   size_t bytes = n * sizeof(double);
   if ((rc = g.scratch_in.ensure(bytes)) || (rc = g.scratch_in2.ensure(bytes)) || (rc = g.scratch_out.ensure(bytes)))

@@ -280,7 +280,8 @@ int cpl_debug_fp64_peak(int iters, double* tflops);
  * cos): 29 sin_rn 30 cos_rn 31 acos_rn 32 dsin_rd 33 dsin_ru 34 dcos_rd
  * 35 dcos_ru 36 dacos_rd 37 dacos_ru.  The ordered minimum's bit map
  * (atomic_float.hpp:56-66): 38 the u64 image of a carried in out's bits,
- * 39 the double whose image is the u64 carried in a's bits. */
+ * 39 the double whose image is the u64 carried in a's bits.  The hardware
+ * estimates the division and square root build on: 40 v_rcp_f64, 41 v_rsq_f64. */
 
 /* Elementwise interval operations of the algcuda::Interval<double> surface
  * (interval.cuh) on n intervals; b is the second operand (binary ops; for the
