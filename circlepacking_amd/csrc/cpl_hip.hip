@@ -204,14 +204,19 @@ __device__ __forceinline__ uint32_t wave_slot() { return (blockIdx.x * blockDim.
 // Block-wide sum of per-thread counters c[16] (RoundStats word order) added to
 // *dst with one atomic per nonzero word: waves reduce by shuffles, wave
 // partials meet in LDS.  Called once, at the end of a kernel, by every thread.
+// UNIFORM: c holds wave totals (the same value in every lane), so no
+// reduction across the wave is needed.
+template <bool UNIFORM = false>
 __device__ __forceinline__ void block_stats_add(RoundStats* dst, const unsigned long long* c) {
   __shared__ unsigned long long part[16][16];  // [wave][word], blocks of up to 1024 threads
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     unsigned long long v = c[k];
+    if (!UNIFORM) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
     if (lane == 0) part[w][k] = v;
   }
   __syncthreads();
@@ -277,7 +282,8 @@ __device__ __forceinline__ int split_axis(const RefineParams& P, uint32_t meta, 
 // 64-cell groups with a wave-level grid stride (the operands are ~23 KB per
 // wave; reloading them per group dominated the kernel's time).
 __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, uint64_t n, uint8_t* out) {
-  if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);
+  uint64_t incl = 0;
+  if (P.in_seg) n = seg_scan(P.in_seg, P.seg_cap, incl);
   MlpRegs W;
   mlp_load(P.mlp, W);
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
@@ -285,12 +291,13 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
   for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); grp * 64 < n; grp += waves) {
     const uint64_t i = grp * 64 + lane;
     const bool valid = i < n;
+    const uint64_t a = P.in_seg ? seg_locate_run(grp * 64, i, incl, P.seg_cap) : i;
     double e[8];
     uint32_t meta = 0;
     if (valid) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
-      meta = in.meta[i];
+      for (int k = 0; k < 8; ++k) e[k] = in.e(k, a);
+      meta = in.meta(a);
     }
     float y[5];
     propose(P, W, valid, e, y);
@@ -311,25 +318,32 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
 // 2 % slower with it).
 template <int F, bool ROOT, int LB = 256, bool EXT = false, int WAVES = 3>
 __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in, uint64_t n) {
-  if (P.in_count) n = min((uint64_t)*P.in_count, P.out_cap);  // regions share one capacity; overflow is reported
-  if (P.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *P.zero_count = 0u;
+  uint64_t incl = 0;
+  if (!ROOT && P.in_seg) n = seg_scan(P.in_seg, P.seg_cap, incl);
+  if (P.zero_seg && blockIdx.x == 0 && threadIdx.x < kSegs) P.zero_seg[threadIdx.x * kSegStride] = 0ull;
+  const SegOut so{P.out_seg, P.seg_cap, P.overflow};
+  SegCursor cur = seg_cursor();
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // Wave-uniform counters (kept in scalar registers: counts come from ballots,
   // digests are reduced across the wave each iteration), so no vector
-  // registers are held across the evaluation for statistics.
-  uint64_t c_eval = 0, c_v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c_dig = 0, c_vdig = 0, c_cert = 0, c_unres = 0;
-  uint64_t c_f32 = 0, c_f32t = 0;
+  // registers are held across the evaluation for statistics.  32-bit counts
+  // of the six verdicts and of the split items; evaluations, certified and
+  // unresolved items follow from them at the end (fewer scalar registers live
+  // across the evaluation, which spills scalars into vector lanes otherwise).
+  uint32_t c_v[6] = {0, 0, 0, 0, 0, 0}, c_split = 0, c_f32 = 0, c_f32t = 0;
+  uint64_t c_dig = 0, c_vdig = 0;
 
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
     bool valid = i < n;
+    uint64_t a = i;
+    if (!ROOT && P.in_seg) a = seg_locate_run(base + (threadIdx.x & ~63u), i, incl, P.seg_cap);
     double e[8];
     uint32_t root = 0, la_idx = 0, meta = 0;
     uint8_t vb = 0;
     bool split = false;
     bool hit = false, tried = false;
-    int cert = 0;
     uint64_t h = 0, hv = 0;
     if (valid) {
       if (ROOT) {
@@ -350,10 +364,10 @@ __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in
         meta = 0;
       } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
-        root = in.root[i];
-        la_idx = in.la_idx[i];
-        meta = in.meta[i];
+        for (int k = 0; k < 8; ++k) e[k] = in.e(k, a);
+        root = in.root(a);
+        la_idx = in.la_idx(a);
+        meta = in.meta(a);
       }
     }
     uint8_t pb = kNoProposal;
@@ -376,29 +390,24 @@ __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in
         split = true;
       } else {
         switch (v) {
-          case V_PROVEN: fl |= CPL_RF_PROVEN; cert = 1; break;
-          case V_EMPTY: fl |= CPL_RF_EMPTY; cert = 1; break;
-          case V_OOD: fl |= CPL_RF_OOD; cert = 1; break;
-          case V_CRITICAL: fl |= CPL_RF_CRITICAL; cert = 2; break;
-          case V_NOFIT: fl |= CPL_RF_NOFIT; cert = 2; break;
-          default: fl |= CPL_RF_TRAP; cert = 2; break;
+          case V_PROVEN: fl |= CPL_RF_PROVEN; break;
+          case V_EMPTY: fl |= CPL_RF_EMPTY; break;
+          case V_OOD: fl |= CPL_RF_OOD; break;
+          case V_CRITICAL: fl |= CPL_RF_CRITICAL; break;
+          case V_NOFIT: fl |= CPL_RF_NOFIT; break;
+          default: fl |= CPL_RF_TRAP; break;
         }
       }
       if (fl && P.root_flags) flag_root(P.root_flags, fbyte, fl);
     }
-    // ---- compaction of children: wave ballot + one atomic per wave, issued
-    // before the statistics so its round trip overlaps them ----
-    const uint64_t mask = __ballot(split);
-    const int leader = mask ? __ffsll((unsigned long long)mask) - 1 : 0;
-    uint32_t basepos = 0;
-    if (mask && lane == leader) basepos = atomicAdd(P.out_count, (uint32_t)__popcll(mask) * (uint32_t)P.n_children);
+    // ---- children: one segment append per wave, reserved before the
+    // statistics so the atomic's round trip overlaps them ----
+    const SegTicket tk = seg_reserve(so, cur, split, (uint32_t)P.n_children);
     {
       const int v = vb & 7;
-      c_eval += __popcll(__ballot(valid));
 #pragma unroll
-      for (int k = 0; k < 8; ++k) c_v[k] += __popcll(__ballot(valid && v == k));
-      c_cert += __popcll(__ballot(cert == 1));
-      c_unres += __popcll(__ballot(cert == 2));
+      for (int k = 0; k < 6; ++k) c_v[k] += (uint32_t)__popcll(__ballot(valid && v == k));
+      c_split += (uint32_t)__popcll(__ballot(split));
       if (EXT) {
         c_f32 += __popcll(__ballot(hit));
         c_f32t += __popcll(__ballot(tried));
@@ -414,72 +423,64 @@ __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in
         c_vdig += dv;
       }
     }
-    if (mask) {
-      uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      basepos = __shfl(basepos, leader, 64);
-      if (split) {
-        uint64_t pos = (uint64_t)basepos + (uint64_t)rank * P.n_children;
-        if (pos + P.n_children > P.out_cap) {
-          atomicOr(P.overflow, 1u);
-        } else {
-          int depth = (int)(meta >> 16);
-          uint32_t splits = meta & 0xFFFFu;
-          if (P.split_mode == 0) {
-            const int best = split_axis(P, meta, pb);
-            IV s0 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 0, 2);
-            IV s1 = get_subinterval(IV(e[2 * best], e[2 * best + 1]), 1, 2);
-            uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | (splits + (1u << (4 * best)));
-            for (int c = 0; c < 2; ++c) {
-              uint64_t o = pos + c;
+    if (tk.total) {
+      const int depth = (int)(meta >> 16);
+      const uint32_t splits = meta & 0xFFFFu;
+      if (P.split_mode == 0) {
+        const int best = split ? split_axis(P, meta, pb) : 0;
+        const uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | (splits + (1u << (4 * best)));
+        seg_commit(so, cur, tk, [&](uint32_t c, uint64_t o) {
+          const IV sc = get_subinterval(IV(e[2 * best], e[2 * best + 1]), (int)c, 2);
 #pragma unroll
-              for (int k = 0; k < 8; ++k) P.out.e[k][o] = e[k];
-              P.out.e[2 * best][o] = c ? s1.lb : s0.lb;
-              P.out.e[2 * best + 1][o] = c ? s1.ub : s0.ub;
-              P.out.root[o] = root;
-              P.out.la_idx[o] = best == 0 ? la_idx * 2 + c : la_idx;
-              P.out.meta[o] = cmeta;
-            }
-          } else {
-            // all eligible axes at once: child c takes bit k of c for the k-th eligible axis
-            uint32_t cs = splits;
-            for (int a = 0; a < 4; ++a)
-              if (P.split_axes & (1u << a)) cs += (1u << (4 * a));
-            uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | cs;
-            for (int c = 0; c < P.n_children; ++c) {
-              uint64_t o = pos + c;
-              double ce[8];
+          for (int k = 0; k < 8; ++k) P.out.e(k, o) = e[k];
+          P.out.e(2 * best, o) = sc.lb;
+          P.out.e(2 * best + 1, o) = sc.ub;
+          P.out.root(o) = root;
+          P.out.la_idx(o) = best == 0 ? la_idx * 2 + c : la_idx;
+          P.out.meta(o) = cmeta;
+        });
+      } else {
+        // all eligible axes at once: child c takes bit k of c for the k-th eligible axis
+        uint32_t cs = splits;
+        for (int ax = 0; ax < 4; ++ax)
+          if (P.split_axes & (1u << ax)) cs += (1u << (4 * ax));
+        const uint32_t cmeta = ((uint32_t)(depth + 1) << 16) | cs;
+        const int nbits = __popc(P.split_axes);
+        seg_commit(so, cur, tk, [&](uint32_t c, uint64_t o) {
+          double ce[8];
 #pragma unroll
-              for (int k = 0; k < 8; ++k) ce[k] = e[k];
-              uint32_t cla = la_idx;
-              const int nbits = __popc(P.split_axes);
-              int bit = 0;
-              for (int a = 0; a < 4; ++a) {
-                if (!(P.split_axes & (1u << a))) continue;
-                int side = (c >> (nbits - 1 - bit)) & 1;  // first eligible axis = most significant bit
-                ++bit;
-                IV s = get_subinterval(IV(ce[2 * a], ce[2 * a + 1]), side, 2);
-                ce[2 * a] = s.lb;
-                ce[2 * a + 1] = s.ub;
-                if (a == 0) cla = cla * 2 + side;
-              }
-#pragma unroll
-              for (int k = 0; k < 8; ++k) P.out.e[k][o] = ce[k];
-              P.out.root[o] = root;
-              P.out.la_idx[o] = cla;
-              P.out.meta[o] = cmeta;
-            }
+          for (int k = 0; k < 8; ++k) ce[k] = e[k];
+          uint32_t cla = la_idx;
+          int bit = 0;
+          for (int ax = 0; ax < 4; ++ax) {
+            if (!(P.split_axes & (1u << ax))) continue;
+            const int side = (int)(c >> (nbits - 1 - bit)) & 1;  // first eligible axis = most significant bit
+            ++bit;
+            const IV sc = get_subinterval(IV(ce[2 * ax], ce[2 * ax + 1]), side, 2);
+            ce[2 * ax] = sc.lb;
+            ce[2 * ax + 1] = sc.ub;
+            if (ax == 0) cla = cla * 2 + side;
           }
-        }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) P.out.e(k, o) = ce[k];
+          P.out.root(o) = root;
+          P.out.la_idx(o) = cla;
+          P.out.meta(o) = cmeta;
+        });
       }
     }
   }
-  // wave totals enter the block sum once (lane 0)
-  const bool l0 = lane == 0;
-  const unsigned long long acc[16] = {l0 ? c_eval : 0,  l0 ? c_v[0] : 0,  l0 ? c_v[1] : 0, l0 ? c_v[2] : 0,
-                                      l0 ? c_v[3] : 0,  l0 ? c_v[4] : 0,  l0 ? c_v[5] : 0, l0 ? c_v[6] : 0,
-                                      l0 ? c_v[7] : 0,  l0 ? c_dig : 0,   l0 ? c_vdig : 0, l0 ? c_cert : 0,
-                                      l0 ? c_unres : 0, l0 ? c_f32 : 0,   l0 ? c_f32t : 0, 0};
-  block_stats_add(stat_slot(P, P.depth, F, blockIdx.x), acc);
+  // wave totals (wave-uniform) enter the block sum; PROVEN, EMPTY and OOD
+  // items are certified, CRITICAL items that were not split, NOFIT and TRAP
+  // items are unresolved
+  uint64_t c_eval = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c_eval += c_v[k];
+  const uint64_t c_cert = (uint64_t)c_v[V_PROVEN] + c_v[V_EMPTY] + c_v[V_OOD];
+  const uint64_t c_unres = (uint64_t)(c_v[V_CRITICAL] - c_split) + c_v[V_NOFIT] + c_v[V_TRAP];
+  const unsigned long long acc[16] = {c_eval, c_v[0], c_v[1], c_v[2], c_v[3], c_v[4], c_v[5], 0,
+                                      0,      c_dig,  c_vdig, c_cert, c_unres, c_f32, c_f32t, 0};
+  block_stats_add<true>(stat_slot(P, P.depth, F, blockIdx.x), acc);
 }
 
 // Root round with admissibility decided once per (la, r1, r2) triple: an
@@ -493,6 +494,8 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
   const uint64_t n_tri = (uint64_t)P.grid[0] * P.grid[1] * P.grid[2];
   const int n3 = P.grid[3];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const SegOut so{P.out_seg, P.seg_cap, P.overflow};
+  SegCursor cur = seg_cursor();
   unsigned long long c_ood = 0, c_dig = 0, c_vdig = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_tri; base += stride) {
     uint64_t t = base + threadIdx.x;
@@ -539,44 +542,29 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
         }
       }
     }
-    uint64_t mask = __ballot(live);
-    if (mask) {
-      uint32_t cnt = (uint32_t)__popcll(mask);
-      uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      int leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t basepos = 0;
-      if (lane == leader) basepos = atomicAdd(P.out_count, cnt * (uint32_t)n3);
-      basepos = __shfl(basepos, leader, 64);
-      if (live) {
-        uint64_t pos = (uint64_t)basepos + (uint64_t)rank * n3;
-        if (pos + n3 > P.out_cap) {
-          atomicOr(P.overflow, 1u);
-        } else {
-          for (int k = 0; k < n3; ++k) {
-            IV p = get_subinterval(IV(P.box_lb[3], P.box_ub[3]), k, n3);
-            uint64_t o = pos + k;
+    const SegTicket tk = seg_reserve(so, cur, live, (uint32_t)n3);
+    if (tk.total)
+      seg_commit(so, cur, tk, [&](uint32_t k, uint64_t o) {
+        const IV p = get_subinterval(IV(P.box_lb[3], P.box_ub[3]), (int)k, n3);
 #pragma unroll
-            for (int j = 0; j < 6; ++j) P.out.e[j][o] = e[j];
-            P.out.e[6][o] = p.lb;
-            P.out.e[7][o] = p.ub;
-            P.out.root[o] = (uint32_t)(t * n3 + k);
-            P.out.la_idx[o] = (uint32_t)i0;
-            P.out.meta[o] = 0u;
-          }
-        }
-      }
-    }
+        for (int j = 0; j < 6; ++j) P.out.e(j, o) = e[j];
+        P.out.e(6, o) = p.lb;
+        P.out.e(7, o) = p.ub;
+        P.out.root(o) = (uint32_t)(t * n3 + k);
+        P.out.la_idx(o) = (uint32_t)i0;
+        P.out.meta(o) = 0u;
+      });
   }
   unsigned long long acc[16] = {c_ood, 0, 0, 0, 0, 0, 0, 0, 0, c_dig, c_vdig, c_ood, 0, 0, 0, 0};
   acc[1 + V_OOD] = c_ood;
   block_stats_add(stat_slot(P, 0, F, blockIdx.x), acc);
 }
 
-// Zero family F's statistics records (every round and slot) and the five
-// counters and the overflow word of family slot `slot`: the head of that
-// family's captured chain in the graph form, so the chains need no common
-// prologue (and no cross-stream wait) before they start.
-__global__ void k_clear_family(RoundStats* stats, int F, uint32_t* counters, int slot) {
+// Zero family F's statistics records (every round and slot), the segment
+// counts of the five worklist regions and the overflow word of family slot
+// `slot`: the head of that family's captured chain in the graph form, so the
+// chains need no common prologue (and no cross-stream wait) before they start.
+__global__ void k_clear_family(RoundStats* stats, int F, unsigned long long* seg_counts, uint32_t* overflow, int slot) {
   constexpr int W = sizeof(RoundStats) / sizeof(unsigned long long);
   constexpr int per_round = STAT_SLOTS * W;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -584,7 +572,17 @@ __global__ void k_clear_family(RoundStats* stats, int F, uint32_t* counters, int
     const int d = i / per_round, k = i % per_round;
     reinterpret_cast<unsigned long long*>(stats + (size_t)(d * CPL_NUM_FAMILIES + F) * STAT_SLOTS)[k] = 0ull;
   }
-  if (i < 6) counters[i * CPL_NUM_FAMILIES + slot] = 0u;  // counter buffers 0..4, overflow word (5)
+  if (i < 5 * kSegs)
+    seg_counts[((size_t)(i / kSegs) * CPL_NUM_FAMILIES + slot) * kSegs * kSegStride + (size_t)(i % kSegs) * kSegStride] = 0ull;
+  if (i == 0) overflow[slot] = 0u;
+}
+
+// Item totals (sum of the clamped segment counts) of the regions of `nf`
+// family slots whose counts start at `seg_counts`; one wave per slot.
+__global__ void k_seg_totals(const unsigned long long* seg_counts, uint32_t seg_cap, uint64_t* totals) {
+  uint64_t incl;
+  const uint64_t n = seg_scan(seg_counts + (size_t)blockIdx.x * kSegs * kSegStride, seg_cap, incl);
+  if (threadIdx.x == 0) totals[blockIdx.x] = n;
 }
 
 // Sum the STAT_SLOTS copies of every (round, family) record into out.
@@ -622,7 +620,7 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 struct PersistParams {
   RefineParams R;
   Worklist roots;              // live root cells of this family (depth 0)
-  const uint32_t* n_roots;     // their count
+  const unsigned long long* root_seg;  // their segment counts
   uint32_t* q;                 // [0] root claim counter, [4] error flags
   double* sp_e;                // private spill stacks: [wave][8][spill_cap]
   uint32_t* sp_u;              // [wave][3][spill_cap]
@@ -699,10 +697,10 @@ __device__ __forceinline__ void load_item(const RefineParams& P, const Worklist&
     meta = 0;
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[k] = in.e[k][i];
-    root = in.root[i];
-    la_idx = in.la_idx[i];
-    meta = in.meta[i];
+    for (int k = 0; k < 8; ++k) e[k] = in.e(k, i);
+    root = in.root(i);
+    la_idx = in.la_idx(i);
+    meta = in.meta(i);
   }
 }
 
@@ -715,11 +713,16 @@ __device__ __forceinline__ double item_lbrsp(const RefineParams& P, uint32_t met
 // settled fraction of the children of each eligible axis.
 template <int F, bool ROOT>
 __global__ __launch_bounds__(256) void k_record(RefineParams P, Worklist in, uint64_t n, RecordSink S) {
+  uint64_t incl = 0;
+  if (!ROOT && P.in_seg) n = seg_scan(P.in_seg, P.seg_cap, incl);
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t a = (!ROOT && P.in_seg) ? seg_locate_run(base + (threadIdx.x & ~63u), i, incl, P.seg_cap) : i;
+    if (i >= n) continue;
     double e[8];
     uint32_t root, la_idx, meta;
-    load_item<ROOT>(P, in, i, e, root, la_idx, meta);
+    load_item<ROOT>(P, in, a, e, root, la_idx, meta);
     if (S.keep > 1 && mix64(item_hash(e, root, F, meta)) % S.keep != 0) continue;
     const int depth = (int)(meta >> 16);
     const double lbrsp = item_lbrsp(P, meta, la_idx);
@@ -774,7 +777,8 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
   uint32_t* pu = Q.sp_u + (size_t)blockIdx.x * 3 * SC;
   const int NCH = P.n_children;
   const int S = 64 * NCH;        // most cells one batch can push
-  const uint32_t n_roots = (uint32_t)min((uint64_t)*Q.n_roots, P.out_cap);  // overflow is reported by the host
+  uint64_t root_incl;
+  const uint32_t n_roots = (uint32_t)seg_scan(Q.root_seg, P.seg_cap, root_incl);  // overflow is reported by the host
   int count = 0;                 // cells in LDS (wave-uniform)
   uint32_t spilled = 0;          // cells in the private HBM stack (wave-uniform, multiple of 64)
   bool roots_left = true;
@@ -794,12 +798,12 @@ __global__ __launch_bounds__(64) void k_persist(PersistParams Q) {
       r0 = __shfl(r0, 0, 64);
       if (r0 >= n_roots) { roots_left = false; break; }
       const uint32_t take = min(want, n_roots - r0);
+      const uint64_t i = seg_locate(r0 + min((uint32_t)lane, take - 1), root_incl, P.seg_cap);
       if ((uint32_t)lane < take) {
-        const uint32_t i = r0 + lane;
-        for (int k = 0; k < 8; ++k) se[k * CAP + count + lane] = Q.roots.e[k][i];
-        su[0 * CAP + count + lane] = Q.roots.root[i];
-        su[1 * CAP + count + lane] = Q.roots.la_idx[i];
-        su[2 * CAP + count + lane] = Q.roots.meta[i];
+        for (int k = 0; k < 8; ++k) se[k * CAP + count + lane] = Q.roots.e(k, i);
+        su[0 * CAP + count + lane] = Q.roots.root(i);
+        su[1 * CAP + count + lane] = Q.roots.la_idx(i);
+        su[2 * CAP + count + lane] = Q.roots.meta(i);
       }
       count += (int)take;
     }
@@ -1303,11 +1307,9 @@ Worklist region(DevBuf& b, uint64_t cap, int slot) {
   const size_t per_item = 8 * sizeof(double) + 3 * sizeof(uint32_t);
   char* base = b.as<char>() + (size_t)slot * cap * per_item;
   Worklist w;
-  for (int k = 0; k < 8; ++k) w.e[k] = reinterpret_cast<double*>(base + (size_t)k * cap * sizeof(double));
-  char* u = base + 8 * cap * sizeof(double);
-  w.root = reinterpret_cast<uint32_t*>(u);
-  w.la_idx = reinterpret_cast<uint32_t*>(u + cap * sizeof(uint32_t));
-  w.meta = reinterpret_cast<uint32_t*>(u + 2 * cap * sizeof(uint32_t));
+  w.ev = reinterpret_cast<double*>(base);
+  w.uv = reinterpret_cast<uint32_t*>(base + 8 * cap * sizeof(double));
+  w.cap = cap;
   return w;
 }
 
@@ -1508,7 +1510,8 @@ struct Plan {
   uint64_t n_roots = 1;
   uint64_t n_triples = 1;
   uint64_t la_entries = 0;
-  uint64_t cap = 0;
+  uint64_t cap = 0;       // items per family region
+  uint32_t seg_cap = 0;   // items per segment: cap / kSegs
   bool digest = true;
   bool flags = false;
   bool ext = false;       // extended kernels (FP32 prefilter)
@@ -1539,6 +1542,9 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.la_entries = (uint64_t)cfg->grid[0] * ((1ull << pl.la_levels) - 1);
   const char* cap_env = getenv("CPL_WORKLIST_CAP");
   pl.cap = cap_env ? strtoull(cap_env, nullptr, 10) : (1ull << 23);
+  if (pl.cap < (uint64_t)kSegs || pl.cap > (1ull << 31))
+    return fail(CPL_ERR_ARG, "CPL_WORKLIST_CAP must be between 64 and 2^31 items");
+  pl.seg_cap = (uint32_t)(pl.cap / kSegs);
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
   pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT)) != 0;
@@ -1555,9 +1561,13 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   return CPL_OK;
 }
 
-// Counter words: sets 0, 1, 4 round counters, set 2 live root items, then the
-// overflow word.
-constexpr size_t kCounterBytes = 6 * CPL_NUM_FAMILIES * sizeof(uint32_t) + 64;
+// Counter buffer: the segment counts of five worklist regions per family slot
+// (sets 0, 1, 4: the round regions of the graph path's rotation, set 2: live
+// root items, set 3 unused), then one overflow word per family slot and the
+// plain path's item totals.
+constexpr size_t kSegSetWords = (size_t)kSegs * kSegStride;  // u64 words per region
+constexpr size_t kSegCountBytes = 5 * CPL_NUM_FAMILIES * kSegSetWords * sizeof(unsigned long long);
+constexpr size_t kCounterBytes = kSegCountBytes + 256;
 
 // Device buffers: worklists [0], [1] (ping-pong) and [2] (live root items),
 // each with one region per family; counters[5][family slots] + one overflow
@@ -1578,13 +1588,16 @@ int ensure_buffers(const Plan& pl) {
   return CPL_OK;
 }
 
-uint32_t* counter(int buf, int slot) { return g.counters.as<uint32_t>() + buf * CPL_NUM_FAMILIES + slot; }
-uint32_t* overflow_word(int slot = 0) { return g.counters.as<uint32_t>() + 5 * CPL_NUM_FAMILIES + slot; }
+unsigned long long* counter(int buf, int slot) {
+  return g.counters.as<unsigned long long>() + ((size_t)buf * CPL_NUM_FAMILIES + slot) * kSegSetWords;
+}
+uint32_t* overflow_word(int slot = 0) { return reinterpret_cast<uint32_t*>(g.counters.as<char>() + kSegCountBytes) + slot; }
+uint64_t* item_totals() { return reinterpret_cast<uint64_t*>(g.counters.as<char>() + kSegCountBytes + 64); }
 double* la_copy(const Plan& pl, int slot) { return g.la_table.as<double>() + (size_t)slot * pl.la_entries * 3; }
 // Round counters of the graph path: round d reads rot_counter(d) and writes
 // rot_counter(d + 1), and clears rot_counter(d + 2) for the next round (it was
 // the input count of round d - 1), so no clearing node runs between rounds.
-uint32_t* rot_counter(int d, int slot) {
+unsigned long long* rot_counter(int d, int slot) {
   static const int set[3] = {0, 1, 4};
   return counter(set[d % 3], slot);
 }
@@ -1608,7 +1621,7 @@ RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   P.root_flags = pl.flags ? g.root_flags.as<uint32_t>() : nullptr;
   P.stats = g.stats.as<RoundStats>();
   P.overflow = overflow_word();
-  P.out_cap = pl.cap;
+  P.seg_cap = pl.seg_cap;
   P.digest = pl.digest ? 1 : 0;
   P.prefilter_depths = pl.prefilter_depths;
   P.mlp_threshold = cfg->learned_threshold;
@@ -1622,7 +1635,7 @@ void set_family(RefineParams& P, const Plan& pl, int s) {
   P.mlp = pl.learned ? g.mlp.as<MlpPacked>() + pl.fams[s] : nullptr;
 }
 
-// Proposal bytes of the items of `in` (n, or *P.in_count when set) into the
+// Proposal bytes of the items of `in` (n, or the counts P.in_seg when set) into the
 // family's proposal region; the round launched next reads them through P.prop.
 // n_expect: the round's item count (or its expected count for device-sized
 // rounds); the grid gives each wave about kProposeGroups 64-cell groups.
@@ -1710,7 +1723,7 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
                      cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, g.la_table.as<double>());
   HIP_TRY(hipGetLastError());
   RefineParams P = base_params(cfg, pl);
-  std::vector<uint32_t> cur_n(pl.nf, 0);
+  std::vector<uint64_t> cur_n(pl.nf, 0);
   uint64_t peak = 0;
   int cur = 0;
   for (int d = 0; d <= cfg->max_depth; ++d) {
@@ -1723,7 +1736,8 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
       any = true;
       set_family(P, pl, s);
       P.out = region(g.wl[nxt], pl.cap, s);
-      P.out_count = counter(nxt, s);
+      P.out_seg = counter(nxt, s);
+      P.in_seg = d == 0 ? nullptr : counter(cur, s);
       Worklist in = region(g.wl[cur], pl.cap, s);
       hipEvent_t a, b;
       HIP_TRY(hipEventCreate(&a));
@@ -1743,12 +1757,15 @@ int refine_plain(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
       kslot.push_back(d * CPL_NUM_FAMILIES + pl.fams[s]);
     }
     if (!any) break;
-    uint32_t h[CPL_NUM_FAMILIES + 1];
-    HIP_TRY(hipMemcpyAsync(h, counter(nxt, 0), CPL_NUM_FAMILIES * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemcpyAsync(&h[CPL_NUM_FAMILIES], overflow_word(), sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipMemsetAsync(counter(cur, 0), 0, CPL_NUM_FAMILIES * sizeof(uint32_t), g.stream));
+    uint64_t h[CPL_NUM_FAMILIES];
+    uint32_t ovf = 0;
+    hipLaunchKernelGGL(k_seg_totals, dim3(pl.nf), dim3(64), 0, g.stream, counter(nxt, 0), pl.seg_cap, item_totals());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h, item_totals(), pl.nf * sizeof(uint64_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemcpyAsync(&ovf, overflow_word(), sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipMemsetAsync(counter(cur, 0), 0, CPL_NUM_FAMILIES * kSegSetWords * sizeof(unsigned long long), g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (h[CPL_NUM_FAMILIES]) return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
+    if (ovf) return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
     uint64_t tot = 0;
     for (int s = 0; s < pl.nf; ++s) {
       cur_n[s] = h[s];
@@ -1856,19 +1873,19 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
   P.overflow = overflow_word(s);
   P.depth = d;
   if (d == 0) {
-    P.in_count = nullptr;
+    P.in_seg = nullptr;
     P.out = region(g.wl[1], pl.cap, s);
-    P.out_count = rot_counter(1, s);
-    P.zero_count = rot_counter(2, s);
+    P.out_seg = rot_counter(1, s);
+    P.zero_seg = rot_counter(2, s);
     P.prop = nullptr;  // the root round runs without proposals in every form
     if (hier) {
       // live root items go to buffer 0, evaluated from there into buffer 1
       RefineParams T = P;
       T.out = region(g.wl[0], pl.cap, s);
-      T.out_count = counter(2, s);
-      T.zero_count = nullptr;
+      T.out_seg = counter(2, s);
+      T.zero_seg = nullptr;
       dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
-      P.in_count = counter(2, s);
+      P.in_seg = counter(2, s);
       dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(0), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0, true);
     } else {
@@ -1879,10 +1896,10 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
   } else {
     // round d reads buffer 1 - d % 2 ... rounds alternate buffers 1 -> 0 -> 1
     const int cur = d % 2 == 1 ? 1 : 0, nxt = cur ^ 1;
-    P.in_count = rot_counter(d, s);
+    P.in_seg = rot_counter(d, s);
     P.out = region(g.wl[nxt], pl.cap, s);
-    P.out_count = rot_counter(d + 1, s);
-    P.zero_count = rot_counter(d + 2, s);
+    P.out_seg = rot_counter(d + 1, s);
+    P.zero_seg = rot_counter(d + 2, s);
     launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, exp_of(d));
     dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d), pl), pl.block, P,
                                   region(g.wl[cur], pl.cap, s), (uint64_t)0, true);
@@ -1894,7 +1911,7 @@ void enqueue_family(const cpl_refine_cfg* cfg, const Plan& pl, int s, const std:
   g_launch_stream = fam_stream_of(s);
   const int words = CPL_MAX_ROUNDS * STAT_SLOTS * (int)(sizeof(RoundStats) / sizeof(unsigned long long));
   hipLaunchKernelGGL(k_clear_family, dim3((words + 255) / 256), dim3(256), 0, launch_stream(), g.stats.as<RoundStats>(),
-                     pl.fams[s], g.counters.as<uint32_t>(), s);
+                     pl.fams[s], counter(0, 0), overflow_word(0), s);
   hipLaunchKernelGGL(k_la_table, dim3((unsigned)((pl.la_entries + 255) / 256)), dim3(256), 0, launch_stream(),
                      cfg->box_lb[0], cfg->box_ub[0], cfg->grid[0], pl.la_levels, cfg->clip, la_copy(pl, s));
   g_launch_stream = nullptr;
@@ -2048,13 +2065,13 @@ int refine_persistent(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stat
     set_family(P, pl, s);
     RefineParams T = P;
     T.out = region(g.wl[0], pl.cap, s);
-    T.out_count = counter(2, s);
+    T.out_seg = counter(2, s);
     dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
     HIP_TRY(hipGetLastError());
     PersistParams Q;
     Q.R = P;
     Q.roots = region(g.wl[0], pl.cap, s);
-    Q.n_roots = counter(2, s);
+    Q.root_seg = counter(2, s);
     Q.q = g.persist_q.as<uint32_t>() + 8 * s;
     char* part = g.wl[1].as<char>() + (size_t)s * grid_max * spill_cap * per_item;
     Q.sp_e = reinterpret_cast<double*>(part);
