@@ -204,19 +204,14 @@ __device__ __forceinline__ uint32_t wave_slot() { return (blockIdx.x * blockDim.
 // Block-wide sum of per-thread counters c[16] (RoundStats word order) added to
 // *dst with one atomic per nonzero word: waves reduce by shuffles, wave
 // partials meet in LDS.  Called once, at the end of a kernel, by every thread.
-// UNIFORM: c holds wave totals (the same value in every lane), so no
-// reduction across the wave is needed.
-template <bool UNIFORM = false>
 __device__ __forceinline__ void block_stats_add(RoundStats* dst, const unsigned long long* c) {
   __shared__ unsigned long long part[16][16];  // [wave][word], blocks of up to 1024 threads
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     unsigned long long v = c[k];
-    if (!UNIFORM) {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0) part[w][k] = v;
   }
   __syncthreads();
@@ -225,6 +220,19 @@ __device__ __forceinline__ void block_stats_add(RoundStats* dst, const unsigned 
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) sum += part[i][threadIdx.x];
     if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(dst) + threadIdx.x, sum);
   }
+}
+
+// Wave totals c[16] (the same value in every lane) added to *dst by lanes
+// 0..15, one word each, with no barrier: a wave that finishes early leaves
+// at once instead of holding its registers until the block's slowest wave
+// reaches a block-wide reduction.
+__device__ __forceinline__ void wave_stats_add(RoundStats* dst, const unsigned long long* c) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long v = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (lane == k) v = c[k];
+  if (lane < 16 && v) atomicAdd(reinterpret_cast<unsigned long long*>(dst) + lane, v);
 }
 
 __device__ __forceinline__ void flag_root(uint32_t* flags, uint64_t byte_index, uint32_t bits) {
@@ -480,7 +488,7 @@ __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in
   const uint64_t c_unres = (uint64_t)(c_v[V_CRITICAL] - c_split) + c_v[V_NOFIT] + c_v[V_TRAP];
   const unsigned long long acc[16] = {c_eval, c_v[0], c_v[1], c_v[2], c_v[3], c_v[4], c_v[5], 0,
                                       0,      c_dig,  c_vdig, c_cert, c_unres, c_f32, c_f32t, 0};
-  block_stats_add<true>(stat_slot(P, P.depth, F, blockIdx.x), acc);
+  wave_stats_add(stat_slot(P, P.depth, F, wave_slot()), acc);
 }
 
 // Root round with admissibility decided once per (la, r1, r2) triple: an
