@@ -99,7 +99,15 @@ __device__ __forceinline__ double step5(double y, bool up) {
 // bound) bring them back.  Zero arguments are common (clamped quotients of the
 // collision angles), and taking them early keeps the stepping loop of step5,
 // which a wave runs whenever one of its lanes needs it, off their path.
+//
+// Wave-uniform fast path: when every lane's argument lies in [2^-1000, 1] (the
+// clamped quotients of the collision angles), asin_rn(x) lies in
+// [2^-1000, pi/2], so step5 is the direct bit step (5 ulps down for the lower,
+// up for the upper bound), the zero and sign clamps cannot apply and only the
+// upper bound's pi/2 clamp can: the same bits as the general path.
+__device__ __forceinline__ bool asin_unit_arg(double x) { return x >= 0x1p-1000 && x <= 1.0; }
 __device__ __forceinline__ double dasin_rd(double x) {
+  if (all_lanes(asin_unit_arg(x))) return dfrom(dbits(asin_rn(x)) - 5);
   if (x == 0.0) return 0.0;
   double y = step5(asin_rn(x), false);
   if (y < -1.5707963267948968) y = -1.5707963267948968;
@@ -107,6 +115,10 @@ __device__ __forceinline__ double dasin_rd(double x) {
   return y;
 }
 __device__ __forceinline__ double dasin_ru(double x) {
+  if (all_lanes(asin_unit_arg(x))) {
+    const double y = dfrom(dbits(asin_rn(x)) + 5);
+    return y > 1.5707963267948968 ? 1.5707963267948968 : y;
+  }
   if (x == 0.0) return 0.0;
   double y = step5(asin_rn(x), true);
   if (y > 1.5707963267948968) y = 1.5707963267948968;

@@ -95,8 +95,19 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
   lc.tighten_lb(sc.lb);
   sc.tighten_ub(lc.ub);
   I result(T(0), T(-1));
-  I lc_sq = lc.square();
-  I sc_sq = sc.square();
+  I lc_sq, sc_sq;
+  if constexpr (std::is_same<T, double>::value) {
+    // both squares in one statement when square()'s own fast case holds for both
+    if (all_lanes(lc.lb >= T(0) && lc.ub > T(0) && sc.lb >= T(0) && sc.ub > T(0))) {
+      mul_pair2(lc.lb, lc.lb, lc.ub, lc.ub, sc.lb, sc.lb, sc.ub, sc.ub, lc_sq.lb, lc_sq.ub, sc_sq.lb, sc_sq.ub);
+    } else {
+      lc_sq = lc.square();
+      sc_sq = sc.square();
+    }
+  } else {
+    lc_sq = lc.square();
+    sc_sq = sc.square();
+  }
   // is_obtuse (collision_angles.cu:29-31) on the squares computed once: the
   // directed-rounding statements are not merged by the compiler
   UB obt = lc_sq > sc_sq + ab.square();
@@ -104,8 +115,13 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
   add_term.tighten_lb(T(0));
 
   if (possibly(obt)) {
-    I lx = T(0.5) * (add_term + ab);
-    I sx = T(0.5) * (add_term - ab);
+    I lx, sx;  // 1/2 (add_term + ab), 1/2 (add_term - ab)
+    if constexpr (std::is_same<T, double>::value) {
+      half_sum_diff(0.5, add_term.lb, add_term.ub, ab.lb, ab.ub, lx.lb, lx.ub, sx.lb, sx.ub);
+    } else {
+      lx = T(0.5) * (add_term + ab);
+      sx = T(0.5) * (add_term - ab);
+    }
     sx.tighten_lb(T(0));
     sx.tighten_ub(sc.ub);
     lx.tighten_ub(lc.ub);
@@ -163,8 +179,13 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
   }
 
   if (!definitely(obt)) {
-    I lx = T(0.5) * (ab + add_term);
-    I sx = T(0.5) * (ab - add_term);
+    I lx, sx;  // 1/2 (ab + add_term), 1/2 (ab - add_term)
+    if constexpr (std::is_same<T, double>::value) {
+      half_sum_diff(0.5, ab.lb, ab.ub, add_term.lb, add_term.ub, lx.lb, lx.ub, sx.lb, sx.ub);
+    } else {
+      lx = T(0.5) * (ab + add_term);
+      sx = T(0.5) * (ab - add_term);
+    }
     sx.tighten_lb(T(0));
     if (sx.empty()) {
       if (!possibly(obt)) { trap = T_ACU_X; return Angle<T>{result, trap}; }

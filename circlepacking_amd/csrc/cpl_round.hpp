@@ -86,6 +86,36 @@ __device__ __forceinline__ void mul_pair(double al, double bl, double au, double
       : "v"(al), "v"(bl), "v"(au), "v"(bu));
 }
 
+// ---- several pairs in one statement -----------------------------------------
+// Each mode switch costs the wave about 30 cycles and four scalar instructions
+// (s_nop, s_setreg, s_nop, s_setreg around the operation); independent
+// directed operations of the same mode share one switch here.  Two or more
+// instructions always separate consecutive s_setreg writes.
+//
+// [rd(a1 b1), ru(c1 d1)] and [rd(a2 b2), ru(c2 d2)]
+__device__ __forceinline__ void mul_pair2(double a1, double b1, double c1, double d1, double a2, double b2, double c2,
+                                          double d2, double& l1, double& u1, double& l2, double& u2) {
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_mul_f64 %0, %4, %5\n\tv_mul_f64 %2, %8, %9\n\t" CPL_MODE_RU
+      "v_mul_f64 %1, %6, %7\n\tv_mul_f64 %3, %10, %11\n\ts_nop 0\n\t" CPL_MODE_RN
+      : "=&v"(l1), "=&v"(u1), "=&v"(l2), "=v"(u2)
+      : "v"(a1), "v"(b1), "v"(c1), "v"(d1), "v"(a2), "v"(b2), "v"(c2), "v"(d2));
+}
+// With h >= 0 (the factor 1/2 of the collision-angle construction):
+//   lx = [rd(h rd(tl + al)), ru(h ru(tu + au))]   (h * (t + a))
+//   sx = [rd(h rd(tl - au)), ru(h ru(tu - al))]   (h * (t - a))
+// the same operations, in the same order per result, as an interval sum and
+// difference followed by interval x scalar.
+__device__ __forceinline__ void half_sum_diff(double h, double tl, double tu, double al, double au, double& lxl,
+                                              double& lxu, double& sxl, double& sxu) {
+  double s0, s1;
+  asm("s_nop 1\n\t" CPL_MODE_RD "v_add_f64 %4, %6, %8\n\tv_add_f64 %5, %6, -%9\n\t"
+      "v_mul_f64 %0, %4, %10\n\tv_mul_f64 %2, %5, %10\n\t" CPL_MODE_RU
+      "v_add_f64 %4, %7, %9\n\tv_add_f64 %5, %7, -%8\n\t"
+      "v_mul_f64 %1, %4, %10\n\tv_mul_f64 %3, %5, %10\n\ts_nop 0\n\t" CPL_MODE_RN
+      : "=&v"(lxl), "=&v"(lxu), "=&v"(sxl), "=&v"(sxu), "=&v"(s0), "=&v"(s1)
+      : "v"(tl), "v"(tu), "v"(al), "v"(au), "v"(h));
+}
+
 // ---- fma with a scalar-register addend --------------------------------------
 // fma(a, b, c) in round-to-nearest (the ambient mode) as the three-operand
 // v_fma_f64 with c in an SGPR pair.  For a constant c the compiler otherwise
