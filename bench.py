@@ -16,7 +16,8 @@ Forms timed (each checked for identical results against the plain path):
   persistent  one persistent kernel per family draining a mixed-depth stack
   +fp32       the fastest batched form with the FP32 outward prefilter
   +learned    graph + FP32 prefilter gated by the learned proposal (MLP
-              trained on the GPU from records of a 40^4 run of the same box)
+              trained on the GPU from every record of the same 64^4 run,
+              about 9.6e6, one per criteria evaluation)
 The learned split-axis policy changes the set of evaluated cells; it is timed
 and reported separately (evaluations, unresolved leaves), not as `value`.
 
@@ -36,7 +37,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cells certified/s; time-to-proof, tuned vs plain FP64 GPU, identical verdicts"
 FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_round_traffic.json")
+# Per-launch counter averages of the round kernels (rocprofv3 --pmc passes over
+# `bench.py --traffic-probe`, summarised by `bench.py --summarize-pmc`).
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_round_counters_4param_64grid_depth6.json")
+VALU_ISSUE_HZ = 1024 * 2.4e9  # 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD per 4 cycles at 2.4 GHz -> /4 below
+FP64_LANE_OPS = 39.3e12       # 256 CUs x 64 FP64 lanes x 2.4 GHz (an fma counts once)
 
 
 def kernel_groups(stats, cfg):
@@ -105,6 +110,56 @@ def reference_grid_launches(cpl, abi):
     return reference_grid
 
 
+def summarize_pmc(dirs):
+    """Per-launch averages of every counter of every kernel in the rocprofv3
+    counter CSVs under `dirs` (a counter is summed over its dimensions within a
+    dispatch, then averaged over the dispatches of the kernel).  FETCH_SIZE and
+    WRITE_SIZE are calibrated against the 256 MiB device copies of --traffic-probe
+    (the `__amd_rocclr_copyBuffer` dispatches of the largest grid; the library's
+    own small copies are left out), each of which reads and writes 256 MiB: the
+    scales turn the KiB counts into bytes moved, and `hbm_bytes` is their scaled
+    sum."""
+    import csv
+    import glob
+
+    per_dispatch = {}  # (file, dispatch) -> {"name":..., counter: value}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as fh:
+                for row in csv.DictReader(fh):
+                    key = (path, int(row["Dispatch_Id"]))
+                    e = per_dispatch.setdefault(key, {"name": row["Kernel_Name"], "grid": int(row["Grid_Size"])})
+                    e[row["Counter_Name"]] = e.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    copies = [e for e in per_dispatch.values() if e["name"].startswith("__amd_rocclr_copyBuffer")]
+    big = max((e["grid"] for e in copies), default=0)
+    sums, counts = {}, {}
+    for e in per_dispatch.values():
+        name = e["name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+        if name == "__amd_rocclr_copyBuffer" and e["grid"] == big:
+            name = "__amd_rocclr_copyBuffer_256MiB"
+        acc = sums.setdefault(name, {})
+        cnt = counts.setdefault(name, {})
+        for k, v in e.items():
+            if k in ("name", "grid"):
+                continue
+            acc[k] = acc.get(k, 0.0) + v
+            cnt[k] = cnt.get(k, 0) + 1
+    out = {n: {k: acc[k] / counts[n][k] for k in acc} for n, acc in sums.items()}
+    out = {n: dict(v, launches=max(counts[n].values())) for n, v in out.items()}
+    copy = out.get("__amd_rocclr_copyBuffer_256MiB", {})
+    copy_kib = 256 * 1024.0
+    cal = {"copy_bytes": 256 << 20}
+    if copy.get("FETCH_SIZE"):
+        cal["fetch_scale"] = copy_kib / copy["FETCH_SIZE"]
+    if copy.get("WRITE_SIZE"):
+        cal["write_scale"] = copy_kib / copy["WRITE_SIZE"]
+    for v in out.values():
+        if "FETCH_SIZE" in v and "WRITE_SIZE" in v and "fetch_scale" in cal and "write_scale" in cal:
+            v["hbm_bytes"] = 1024.0 * (v["FETCH_SIZE"] * cal["fetch_scale"] + v["WRITE_SIZE"] * cal["write_scale"])
+    out["_calibration"] = cal
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +184,19 @@ def main():
                     help="only the one-launch-per-family timing on the reference grids (for rocprofv3), then exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run a 256 MiB device copy three times and one plain run, then exit (for rocprofv3 --pmc)")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR",
+                    help="rocprofv3 --pmc output directories (csv) of --traffic-probe runs: write the per-launch "
+                         "counter averages of every kernel as JSON to --out, then exit (no GPU needed)")
+    ap.add_argument("--out", default=None, help="output file of --summarize-pmc")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summary = summarize_pmc(args.summarize_pmc)
+        text = json.dumps(summary, indent=1, sort_keys=True)
+        if args.out:
+            with open(args.out, "w") as fh:
+                fh.write(text + "\n")
+        print(text)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the proof over several GPUs is not built", file=sys.stderr)
@@ -213,8 +280,10 @@ def main():
     learned_info = None
     if not args.no_learned and "graph" in forms:
         t0 = time.perf_counter()
-        train_cfg = abi.cfg_4param(grid=(40, 40, 40, 40), max_depth=args.depth)
-        models = learned.train(cpl, train_cfg, families=base.families(), epochs=12, batch=1024)
+        # every record of the benchmark's own 64^4 run (about 9.6e6, one per
+        # criteria evaluation), 8 epochs of Adam on the GPU
+        train_cfg = abi.cfg_4param(n=args.grid, max_depth=args.depth)
+        models = learned.train(cpl, train_cfg, families=base.families(), max_records=10_000_000, epochs=8, batch=4096)
         learned_info = {"train_seconds": time.perf_counter() - t0,
                         "records": {abi.FAMILY_NAMES[f]: m[1]["records"] for f, m in models.items()},
                         "final_loss": {abi.FAMILY_NAMES[f]: (m[1].get("loss") or [None])[-1] for f, m in models.items()}}
@@ -295,11 +364,19 @@ def main():
             flops_per_launch = sum(float(np.dot(c, flops_per_eval[f])) for _, _, c in launches) / len(launches)
             achieved = flops_per_launch / avg_sec / 1e12
         items = sum(float(c.sum()) for _, _, c in launches) / len(launches)
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            with open(TRAFFIC_FILE) as fh:
-                t = json.load(fh)
-            traffic = t.get(kname)
+        traffic = fp64_util = valu_util = None
+        if os.path.exists(COUNTERS_FILE):
+            with open(COUNTERS_FILE) as fh:
+                c = json.load(fh).get(kname, {})
+            traffic = c.get("hbm_bytes")
+            fp64 = sum(c.get(k, 0.0) for k in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                                                "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64"))
+            if fp64:
+                # wave instructions x 64 lanes against the FP64 lane-op peak
+                fp64_util = fp64 * 64 / (avg_sec * FP64_LANE_OPS)
+            if c.get("SQ_INSTS_VALU"):
+                # every wave64 VALU instruction occupies its SIMD's VALU for 4 cycles
+                valu_util = c["SQ_INSTS_VALU"] * 4 / (avg_sec * VALU_ISSUE_HZ)
         roof = {
             "bound": "fp64_valu",
             "pipe": "FP64 VALU (directed-rounding interval arithmetic; published FP64 vector peak)",
@@ -314,6 +391,8 @@ def main():
             "frac": (achieved / FP64_PEAK_TFLOPS) if achieved is not None else None,
             "traffic": traffic,
             "traffic_gbps": (traffic / avg_sec / 1e9) if traffic is not None else None,
+            "fp64_valu_util": fp64_util,
+            "valu_issue_util": valu_util,
             "worklist_items_per_launch": items,
         }
 

@@ -82,6 +82,27 @@ def build_driver(verbose=False):
     return PROVER
 
 
+COST_PROBE = os.path.join(HERE, "cpl_cost_probe")
+
+
+def build_cost_probe(verbose=False, occupancy=None, out=None):
+    """The issue-cost probe of the criteria (cost_probe.hip), a standalone HIP
+    program; occupancy: waves per SIMD of its criteria throughput kernel."""
+    out = out or COST_PROBE
+    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")] + [
+        "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", os.path.join(CSRC, "cost_probe.hip")]
+    if occupancy:
+        cmd.insert(1, f"-DCPL_PROBE_OCC={int(occupancy)}")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed on cost_probe.hip ({res.returncode}):\n{res.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))
     print(build_driver(verbose=True))
+    print(build_cost_probe(verbose=True))

@@ -47,6 +47,22 @@ def test_full_grid_batched_forms_match_plain(cpl, opts):
         same(cpl.refine_run(cfg), plain)
 
 
+def test_full_grid_depth8_forms_match_plain(cpl):
+    """64^4 roots refined to depth 8 (about 9.2e7 evaluations, 8.8e6 items in
+    the widest round): graph-captured and persistent forms equal the plain path.
+    The unresolved leaves roughly double per extra level (2.8e6, 4.4e6, 8.8e6 at
+    depths 6, 7, 8) while the certified count barely moves: those cells fail the
+    criteria at every width."""
+    base = abi.cfg_4param(max_depth=8)
+    plain = cpl.refine_run(base)
+    s = plain[0]
+    assert s.rounds == 9 and s.unresolved > 2 * 4418579 - 100000
+    for opts in (G | H, PER | H):
+        cfg = abi.with_options(base, opts)
+        for _ in range(2):
+            same(cpl.refine_run(cfg), plain)
+
+
 def test_persistent_small_capacity_spills_and_matches(cpl, monkeypatch):
     """A deep 2x2 refinement forces the per-wave stacks to spill chunks to the global queue."""
     base = abi.cfg_small_2param(raw=True, n=64, max_depth=6)

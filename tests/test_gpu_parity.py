@@ -64,3 +64,15 @@ def test_4param_full_grid_root_sample(cpl, cplo):
     assert np.array_equal(gf[::stride], of)
     assert g.certified + g.unresolved == sum(
         g.evaluations[d] - (g.evaluations[d + 1] // 2 if d + 1 < g.rounds else 0) for d in range(g.rounds))
+
+
+def test_4param_full_grid_every_64th_root(cpl, cplo):
+    """Full 64^4 grid, depth 6: the per-(root, family) outcome flags of every 64th
+    root (262,144 roots, each refined to its leaves by the oracle) equal the GPU's."""
+    cfg = abi.cfg_4param(max_depth=6)
+    _, gf = cpl.refine_run(cfg)
+    _, of = cplo.refine_run(cfg, root_stride=64)
+    assert of.shape[0] == gf.shape[0] // 64
+    assert np.array_equal(gf[::64], of)
+    # the sample holds roots with criteria-decided leaves of both outcomes
+    assert np.any(of & abi.RF_PROVEN) and np.any(of & abi.RF_CRITICAL)
