@@ -5,9 +5,11 @@
 // not reproduced: asin_rn is an independent design (argument reduction to
 // [0, 1/2] plus a degree-12 Chebyshev polynomial in x^2; coefficients from
 // gen_asin_poly.py), written only with correctly rounded IEEE operations in
-// round-to-nearest (+, -, *, /, fma, sqrt) so its bits are reproducible on any
-// IEEE machine.  Measured max error 1.10 ulp over dense sweeps of [0, 1];
-// the five-ulp widening therefore still encloses the true arcsine.
+// round-to-nearest (+, -, *, fma, sqrt) so its bits are reproducible on any
+// IEEE machine.  Measured max error 1.50 ulp over dense sweeps of [0, 1]
+// (near x = 0.506, where the rounding error of sqrt(t) is not corrected); the
+// five-ulp widening encloses the true arcsine with room for a step across a
+// binade edge (5 > 2 x 1.5 + 1).
 #pragma once
 
 #include "cpl_iv.hpp"
@@ -64,9 +66,10 @@ __device__ __forceinline__ double asin_rn(double x) {
   if (small) {
     r = __builtin_fma(ax * z, p, ax);
   } else {
+    // asin(sqrt t) ~= s + w; the correction of s's own rounding error, (t - s^2) / 2s,
+    // would cost an IEEE division for at most 0.4 ulp of the result
     double s = sqrt_quarter(t);
-    double c = t > 0.0 ? __builtin_fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
-    double w = __builtin_fma(s * t, p, c);
+    double w = (s * t) * p;
     r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));
   }
   return __builtin_copysign(r, x);

@@ -305,18 +305,17 @@ static inline double asin_poly_eval(double z) {
 static inline double asin_rn(double x) {
   double ax = std::fabs(x);
   double r;
-  // 12 fma of the polynomial + 3 (small branch) / 16 (reduced branch)
-  CPLO_FLOPS(ax <= 0.5 ? 24 + 4 : 24 + 18);
+  // the polynomial's 12 fma (24) + 4 (small branch) / 13 (reduced branch)
+  CPLO_FLOPS(ax <= 0.5 ? 24 + 4 : 24 + 13);
   if (ax <= 0.5) {
     double z = ax * ax;
     double p = asin_poly_eval(z);
     r = std::fma(ax * z, p, ax);
   } else {
     double t = 0.5 - 0.5 * ax;           // exact for ax in [0.5, 1]
-    double s = std::sqrt(t);
-    double c = t > 0.0 ? std::fma(-s, s, t) / (s + s) : 0.0;  // sqrt(t) ~= s + c
+    double s = std::sqrt(t);             // its rounding error is not corrected (max 1.50 ulp overall)
     double p = asin_poly_eval(t);
-    double w = std::fma(s * t, p, c);    // asin(s) ~= s + w
+    double w = (s * t) * p;              // asin(s) ~= s + w
     r = asin_pio2_hi - ((s + s) + (w + w - asin_pio2_lo));
   }
   return std::copysign(r, x);
