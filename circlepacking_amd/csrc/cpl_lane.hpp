@@ -166,11 +166,30 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
             trap = T_ASIN;
             return Angle<T>{result, trap};
           }
-          // result = asin(lquot) - asin(squot), lower bound clamped at 0
-          const bool full = W == 3 || !definitely(obt);
+          // result = asin(lquot) - asin(squot), lower bound clamped at 0.  With
+          // A = dasin_rd(lq.lb), B = dasin_ru(lq.ub), C = dasin_ru(sq.ub),
+          // D = dasin_rd(sq.lb): the widened arcsines enclose the monotone arcsine,
+          // so A <= B and D <= C, rd(A - C) <= ru(B - D), and after tighten_lb(0)
+          // the result is empty iff ru(B - D) < 0.  When the acute branch follows
+          // (obtuseness undecided), that emptiness decides between taking and
+          // joining the acute result, so the endpoint no caller reads is needed
+          // only for it: not at all for an upper-only call (its lower bound is left
+          // at -inf, clamped to 0), and for a lower-only call only when
+          // lq.ub < sq.lb (otherwise B >= asin(lq.ub) >= asin(sq.lb) >= D and
+          // ru >= 0; its upper bound is left at +inf).  Read endpoints and every
+          // emptiness decision are those of the full computation.
           T rl = T(0), ru = T(0);
-          if ((W & 1) || full) rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
-          if ((W & 2) || full) ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
+          const T inf = T(__builtin_huge_val());
+          if constexpr (W == 3) {
+            rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
+            ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
+          } else if constexpr (W == 1) {
+            rl = sub_rd(dasin_rd(lquot.lb), dasin_ru(squot.ub));
+            if (!definitely(obt)) ru = lquot.ub < squot.lb ? sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb)) : inf;
+          } else {
+            ru = sub_ru(dasin_ru(lquot.ub), dasin_rd(squot.lb));
+            if (!definitely(obt)) rl = -inf;
+          }
           result = I(rl, ru);
           result.tighten_lb(T(0));
         }
