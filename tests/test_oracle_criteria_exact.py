@@ -182,3 +182,23 @@ def test_reference_grid_output_mode(cplo, ref, family):
         assert [tuple(r) for r in want["records"].tolist()] == got["records"]
         lines += len(got["records"])
     assert lines > 0
+
+
+def test_empty_verdict_only_where_the_reference_checks(cplo, ref):
+    """The kernel bodies return EMPTY when a tightened r1 or r2 is empty, in every
+    family; the reference checks this in the end 3-disk family only
+    (end_configuration.cu:232-234) and evaluates the criteria otherwise.  The
+    difference is never visible: every range's upper bound is its parent's
+    (r2 up to r1_.ub, r3 up to r2_.ub), so the tightened intervals of the
+    reference grids and of clipped synthetic cells are not empty.  Checked on
+    the 4-parameter workload (16^4 roots, refined to depth 6) and on cells of
+    the reference grids of every family, at the reference's subdivision counts
+    and at coarse ones."""
+    st, _ = cplo.refine_run(abi.cfg_4param(grid=(16, 16, 16, 16)), want_flags=False)
+    fv = st.as_dict()["family_verdicts"]
+    assert sum(fv[r][f][abi.V_EMPTY] for r in range(st.rounds) for f in range(6)) == 0
+    assert sum(fv[r][f][abi.V_PROVEN] for r in range(st.rounds) for f in range(6)) > 0
+    for family in FAMILIES:
+        for k, sd in enumerate([abi.REF_SUBDIV[family], (6, 6, 6, 3)]):
+            kinds = cplo.eval_kernel_body(family, grid_cells(ref, family, 2000, 70 + 7 * family + k, sd)) & 7
+            assert abi.V_EMPTY not in set(kinds.tolist()), (family, sd)
