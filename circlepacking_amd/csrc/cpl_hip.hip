@@ -504,7 +504,7 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const SegOut so{P.out_seg, P.seg_cap, P.overflow};
   SegCursor cur = seg_cursor();
-  unsigned long long c_ood = 0, c_dig = 0, c_vdig = 0;
+  unsigned long long c_ood = 0, c_dig = 0, c_vdig = 0, c_live = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n_tri; base += stride) {
     uint64_t t = base + threadIdx.x;
     bool live = false;
@@ -535,6 +535,7 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
         if (!ood) ood = r3_range<F>(la, r2c).empty();
       }
       live = !ood;
+      if (live) c_live += (unsigned long long)n3;
       if (ood) {
         c_ood += (unsigned long long)n3;
         for (int k = 0; k < n3; ++k) {
@@ -563,7 +564,7 @@ __global__ __launch_bounds__(1024) void k_root_triples(RefineParams P) {
         P.out.meta(o) = 0u;
       });
   }
-  unsigned long long acc[16] = {c_ood, 0, 0, 0, 0, 0, 0, 0, 0, c_dig, c_vdig, c_ood, 0, 0, 0, 0};
+  unsigned long long acc[16] = {c_ood, 0, 0, 0, 0, 0, 0, 0, 0, c_dig, c_vdig, c_ood, 0, 0, 0, c_live};
   acc[1 + V_OOD] = c_ood;
   block_stats_add(stat_slot(P, 0, F, blockIdx.x), acc);
 }
@@ -1180,6 +1181,7 @@ struct Context {
   hipStream_t stream = nullptr;
   hipStream_t fam_stream[CPL_NUM_FAMILIES] = {};   // per-family branches (default priority)
   hipEvent_t fork_ev = nullptr, join_ev[CPL_NUM_FAMILIES] = {};
+  hipEvent_t chain_end[CPL_NUM_FAMILIES] = {};  // graph form: end of each family chain (timed)
   DevBuf scratch_in, scratch_in2, scratch_out, cells, verdicts;
   DevBuf wl[2];           // ping-pong worklists; each holds nfam regions
   DevBuf la_table, root_flags, stats, stats_sum, counters;
@@ -1208,6 +1210,7 @@ int ensure_init() {
   for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
     HIP_TRY(hipStreamCreateWithFlags(&g.fam_stream[f], hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&g.join_ev[f], hipEventDisableTiming));
+    HIP_TRY(hipEventCreate(&g.chain_end[f]));
   }
   HIP_TRY(hipEventCreateWithFlags(&g.fork_ev, hipEventDisableTiming));
   // the device address of this device's copy of the no-allocation scratch
@@ -1341,6 +1344,7 @@ int cpl_shutdown(void) {
   for (int f = 0; f < CPL_NUM_FAMILIES; ++f) {
     (void)hipStreamDestroy(g.fam_stream[f]);
     (void)hipEventDestroy(g.join_ev[f]);
+    (void)hipEventDestroy(g.chain_end[f]);
     g.fam_stream[f] = nullptr;
   }
   (void)hipEventDestroy(g.fork_ev);
@@ -1670,6 +1674,10 @@ template <int F> struct LaunchTriples {
   }
 };
 
+// Root items the triple screening passed on, per family, in the last collected
+// graph-form run (sizes the next capture's depth-0 round).
+uint64_t g_live_items[CPL_NUM_FAMILIES];
+
 int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   std::vector<RoundStats> rs(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES);
   {
@@ -1703,6 +1711,7 @@ int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
       stats->certified += r.certified;
       stats->unresolved += r.unresolved;
       stats->fp32_settled[d][f] = r.fp32_settled;
+      if (d == 0) g_live_items[f] = r.live_items;
       stats->fp32_tried[d][f] = r.fp32_tried;
       stats->total_evaluations += r.evaluations;
     }
@@ -1836,8 +1845,11 @@ struct GraphCache {
   hipGraphExec_t exec[1 + CPL_NUM_FAMILIES] = {};
   hipGraph_t graph[1 + CPL_NUM_FAMILIES] = {};
   int n = 0;
-  // expected items per (round, slot) from the previous replay: sizes the grids
+  // expected items per (round, slot) from the previous replay: sizes the grids;
+  // then, per slot, the live root items of the triple screening
   std::vector<uint32_t> expect;
+  // end of each family chain in the previous replay (ms after its start; 0: not measured)
+  float chain_ms[CPL_NUM_FAMILIES] = {};
   void reset() {
     for (int k = 0; k < 1 + CPL_NUM_FAMILIES; ++k) {
       if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
@@ -1875,6 +1887,12 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
     size_t k = (size_t)dd * CPL_NUM_FAMILIES + s;
     return k < expect.size() && expect[k] ? expect[k] : pl.cap;
   };
+  // items of the depth-0 round after the triple screening (all of round 0's
+  // evaluations before the first replay)
+  auto live_of = [&]() -> uint64_t {
+    size_t k = (size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES + s;
+    return k < expect.size() && expect[k] ? expect[k] : exp_of(0);
+  };
   g_launch_stream = fam_stream_of(s);
   set_family(P, pl, s);
   P.la_table = la_copy(pl, s);
@@ -1894,7 +1912,7 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
       T.zero_seg = nullptr;
       dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
       P.in_seg = counter(2, s);
-      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(0), pl), pl.block, P,
+      dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(live_of(), pl), pl.block, P,
                                     region(g.wl[0], pl.cap, s), (uint64_t)0, true);
     } else {
       Worklist none{};
@@ -1929,6 +1947,9 @@ void enqueue_family(const cpl_refine_cfg* cfg, const Plan& pl, int s, const std:
 int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   bool same = g_graph.valid && g_graph.flags == pl.flags && std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) == 0;
   if (!same) {
+    // chain times of another configuration do not order this one's chains
+    if (std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) != 0)
+      for (float& m : g_graph.chain_ms) m = 0.f;
     g_graph.reset();
     for (int k = 0; k <= pl.nf; ++k) {
       hipStream_t cs = k == 0 ? g.stream : fam_stream_of(k - 1);
@@ -1954,13 +1975,22 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
     HIP_TRY(hipGraphLaunch(g_graph.exec[0], g.stream));
     HIP_TRY(hipEventRecord(g.fork_ev, g.stream));
   }
-  // submit the family chains with the most work (previous replay) first: host
-  // submission of a captured chain takes time, the largest chain bounds the run
+  // Submit first the chain that ended last in the previous replay (host
+  // submission of a captured chain takes tens of microseconds, so later chains
+  // start later); before any replay, the chain with the most work: the items of
+  // its refinement rounds and its live root items (round 0's out-of-domain
+  // roots, screened per triple, cost little).
   std::vector<int> order(pl.nf);
-  std::vector<uint64_t> work(pl.nf, 0);
+  std::vector<double> work(pl.nf, 0.0);
+  bool timed = true;
+  for (int s = 0; s < pl.nf; ++s) timed = timed && g_graph.chain_ms[s] > 0.f;
   for (int s = 0; s < pl.nf; ++s) {
     order[s] = s;
-    for (int d = 0; d < CPL_MAX_ROUNDS; ++d) {
+    if (timed) {
+      work[s] = g_graph.chain_ms[s];
+      continue;
+    }
+    for (int d = 1; d <= CPL_MAX_ROUNDS; ++d) {
       const size_t k = (size_t)d * CPL_NUM_FAMILIES + s;
       if (k < g_graph.expect.size()) work[s] += g_graph.expect[k];
     }
@@ -1969,15 +1999,22 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   for (int s : order) {
     if (s == 0) {
       HIP_TRY(hipGraphLaunch(g_graph.exec[1], g.stream));
+      HIP_TRY(hipEventRecord(g.chain_end[0], g.stream));
       continue;
     }
     if (prologue) HIP_TRY(hipStreamWaitEvent(fam_stream_of(s), g.fork_ev, 0));
     HIP_TRY(hipGraphLaunch(g_graph.exec[1 + s], fam_stream_of(s)));
+    HIP_TRY(hipEventRecord(g.chain_end[s], fam_stream_of(s)));
     HIP_TRY(hipEventRecord(g.join_ev[s], fam_stream_of(s)));
   }
   for (int s = 1; s < pl.nf; ++s) HIP_TRY(hipStreamWaitEvent(g.stream, g.join_ev[s], 0));
   HIP_TRY(hipEventRecord(ev1, g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
+  for (int s = 0; s < pl.nf; ++s) {
+    float m = 0.f;
+    HIP_TRY(hipEventElapsedTime(&m, ev0, g.chain_end[s]));
+    g_graph.chain_ms[s] = m;
+  }
   uint32_t ovfw[CPL_NUM_FAMILIES] = {};
   HIP_TRY(hipMemcpy(ovfw, overflow_word(0), pl.nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
   uint32_t ovf = 0;
@@ -1997,13 +2034,17 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   if (rc) return rc;
   // Record the round sizes of this replay; if they differ from the ones the
   // graph's grids were sized for, re-capture on the next call.
-  std::vector<uint32_t> seen((size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES, 0);
+  std::vector<uint32_t> seen((size_t)(CPL_MAX_ROUNDS + 1) * CPL_NUM_FAMILIES, 0);
   for (int d = 0; d < stats->rounds; ++d)
     for (int s = 0; s < pl.nf; ++s) {
       uint64_t n = 0;
       for (int v = 0; v < 8; ++v) n += stats->family_verdicts[d][pl.fams[s]][v];
       seen[(size_t)d * CPL_NUM_FAMILIES + s] = (uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFu);
     }
+  if (cfg->options & CPL_OPT_HIER_ROOTS)
+    for (int s = 0; s < pl.nf; ++s)
+      seen[(size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES + s] =
+          (uint32_t)std::min<uint64_t>(g_live_items[pl.fams[s]], 0xFFFFFFFFu);
   if (seen != g_graph.expect) {
     g_graph.expect = seen;
     g_graph.valid = false;  // keep the exec until the next capture replaces it

@@ -197,7 +197,7 @@ struct RoundStats {
   unsigned long long unresolved;
   unsigned long long fp32_settled;  // items the FP32 prefilter proved
   unsigned long long fp32_tried;    // items the FP32 prefilter ran on
-  unsigned long long pad;
+  unsigned long long live_items;    // round 0 of the graph form: root items the triple screening passed on
 };
 static_assert(sizeof(RoundStats) == 16 * sizeof(unsigned long long), "16 words, RoundStats word order used by kernels");
 

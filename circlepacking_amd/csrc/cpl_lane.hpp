@@ -288,14 +288,20 @@ __device__ __forceinline__ IVT<T> collision_angle_inner_inner(IVT<T> la, IVT<T> 
 // :176-191 (inner_outer: ca = la + ra, cb = 1 - rb).
 // W: the endpoints the caller reads, as for compute_collision_angle_r (the
 // join combines like endpoints only).
+//
+// The reference's three cases (ca definitely longer, cb definitely longer,
+// either) are two calls at most per lane: the first with the operands in the
+// order of the lane's case, the second, for the undecided lanes that did not
+// trap, with the order swapped and joined.  A wave whose lanes fall in
+// different cases runs the angle kernel at most twice instead of once per
+// case; each lane's calls and their order are the reference's.
 template <int W, class T>
 __device__ __forceinline__ IVT<T> do_collision_angle(IVT<T> ca, IVT<T> cb, IVT<T> ab, int& trap) {
   UB ca_longer = (cb <= ca);
-  if (definitely(ca_longer)) return compute_collision_angle<W>(ca, cb, ab, trap);
-  if (definitely(!ca_longer)) return compute_collision_angle<W>(cb, ca, ab, trap);
-  IVT<T> result = compute_collision_angle<W>(ca, cb, ab, trap);
-  if (trap) return result;
-  result.do_join(compute_collision_angle<W>(cb, ca, ab, trap));
+  const bool cb_first = definitely(!ca_longer);
+  const bool either = !definitely(ca_longer) && !cb_first;
+  IVT<T> result = compute_collision_angle<W>(cb_first ? cb : ca, cb_first ? ca : cb, ab, trap);
+  if (either && !trap) result.do_join(compute_collision_angle<W>(cb, ca, ab, trap));
   return result;
 }
 
