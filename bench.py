@@ -40,6 +40,9 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 # Per-launch counter averages of the round kernels (rocprofv3 --pmc passes over
 # `bench.py --traffic-probe`, summarised by `bench.py --summarize-pmc`).
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_round_counters_4param_64grid_depth6.json")
+# VALU and FP64 instruction totals of one graph-form proof (rocprofv3 --pmc over
+# `bench.py --graph-counters`, summed by `bench.py --summarize-graph-pmc`)
+GRAPH_COUNTERS_FILE = os.path.join(ROOT, "profiles", "graph_counters_4param_64grid_depth6.json")
 VALU_ISSUE_HZ = 1024 * 2.4e9  # 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD per 4 cycles at 2.4 GHz -> /4 below
 FP64_LANE_OPS = 39.3e12       # 256 CUs x 64 FP64 lanes x 2.4 GHz (an fma counts once)
 
@@ -160,6 +163,33 @@ def summarize_pmc(dirs):
     return out
 
 
+GRAPH_RUNS = 5  # proofs run by --graph-counters (each call replays the whole proof once)
+
+
+def summarize_graph_pmc(dirs):
+    """Instruction totals of one graph-form proof: every counter summed over
+    all dispatches of each --pmc pass of `bench.py --graph-counters`, divided
+    by GRAPH_RUNS (each refinement call, the two capturing ones included,
+    executes the proof once; the library's small statistics kernels are
+    included and negligible)."""
+    import csv
+    import glob
+
+    tot = {}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as fh:
+                for row in csv.DictReader(fh):
+                    if row["Kernel_Name"].startswith("__amd_rocclr"):
+                        continue
+                    tot[row["Counter_Name"]] = tot.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    out = {k: v / GRAPH_RUNS for k, v in tot.items()}
+    out["fp64_insts"] = sum(out.get(k, 0.0) for k in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                                                      "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64"))
+    out["runs"] = GRAPH_RUNS
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,8 +217,21 @@ def main():
     ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR",
                     help="rocprofv3 --pmc output directories (csv) of --traffic-probe runs: write the per-launch "
                          "counter averages of every kernel as JSON to --out, then exit (no GPU needed)")
-    ap.add_argument("--out", default=None, help="output file of --summarize-pmc")
+    ap.add_argument("--out", default=None, help="output file of --summarize-pmc / --summarize-graph-pmc")
+    ap.add_argument("--graph-counters", action="store_true",
+                    help="run GRAPH_RUNS graph-form proofs (--graph-shape, default 64,2), then exit (for rocprofv3 --pmc)")
+    ap.add_argument("--summarize-graph-pmc", nargs="+", metavar="DIR",
+                    help="rocprofv3 --pmc output directories of --graph-counters runs: write the instruction totals "
+                         "of one proof as JSON to --out, then exit (no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_graph_pmc:
+        summary = summarize_graph_pmc(args.summarize_graph_pmc)
+        text = json.dumps(summary, indent=1, sort_keys=True)
+        if args.out:
+            with open(args.out, "w") as fh:
+                fh.write(text + "\n")
+        print(text)
+        return
     if args.summarize_pmc:
         summary = summarize_pmc(args.summarize_pmc)
         text = json.dumps(summary, indent=1, sort_keys=True)
@@ -216,6 +259,14 @@ def main():
         cpl.refine_run(abi.with_options(abi.cfg_4param(n=args.grid, max_depth=args.depth), abi.OPT_NO_DIGEST),
                        want_flags=False)
         print(json.dumps({"traffic_probe": True, "copy_bytes": 4 * a.numel()}), flush=True)
+        return
+    if args.graph_counters:
+        b, c = (int(x) for x in (args.graph_shape or "64,2").split(","))
+        cfg = abi.with_options(abi.cfg_4param(n=args.grid, max_depth=args.depth),
+                               abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST, block=b, cpt=c)
+        for _ in range(GRAPH_RUNS):
+            cpl.refine_run(cfg, want_flags=False)
+        print(json.dumps({"graph_counters": True, "runs": GRAPH_RUNS, "block": b, "cells_per_thread": c}), flush=True)
         return
     if args.reference_grid_only:
         print(json.dumps({"reference_grid_launch": reference_grid_launches(cpl, abi)}), flush=True)
@@ -295,10 +346,26 @@ def main():
         }
         ls_cfg = abi.with_options(graph_cfg, G | H | ND | LS)
         ms, st = timed(ls_cfg)
+
+        def volume(stats, cfg):  # unresolved volume, fraction of the box summed over the families
+            return sum(stats.unresolved_volume(cfg).values())
+
+        # the same measure one level deeper without the proposal: the depth the
+        # widest-axis rule needs to leave no more unresolved volume
+        deep = abi.cfg_4param(n=args.grid, max_depth=args.depth + 1)
+        deep_cfg = abi.with_options(deep, G | H | ND, block=graph_cfg.block, cpt=graph_cfg.cells_per_thread)
+        ms_deep, st_deep = timed(deep_cfg)
         learned_info["learned_split"] = {
             "ms": ms, "evaluations": st.total_evaluations, "certified": st.certified, "unresolved": st.unresolved,
+            "unresolved_volume": volume(st, base),
             "plain_evaluations": st_plain.total_evaluations, "plain_unresolved": st_plain.unresolved,
-            "note": "changes the set of evaluated cells (another bisection order); not an identical-verdict form",
+            "plain_unresolved_volume": volume(st_plain, base),
+            "plain_one_level_deeper": {"max_depth": args.depth + 1, "ms": ms_deep,
+                                       "evaluations": st_deep.total_evaluations,
+                                       "unresolved_volume": volume(st_deep, deep)},
+            "note": "changes the set of evaluated cells (another bisection order); not an identical-verdict form. "
+                    "unresolved_volume: volume of the leaves left unresolved, as a fraction of the box, summed over "
+                    "the families",
         }
         for f in range(6):
             cpl.mlp_install(f, None)
@@ -396,6 +463,26 @@ def main():
             "worklist_items_per_launch": items,
         }
 
+    # ---- the whole proof in the reported form: algorithmic flops of every
+    # evaluation over the time to proof (all kernels, overlapped families), and
+    # the VALU issue of the same run from its counter summary ----
+    roof_proof = None
+    if flops_per_eval is not None:
+        total = np.zeros((6, 8))
+        for d in range(st_plain.rounds):
+            total += np.array([[st_plain.family_verdicts[d][f][v] for v in range(8)] for f in range(6)],
+                              dtype=np.float64)
+        proof_flops = float(np.sum(total * flops_per_eval))
+        roof_proof = {"form": name, "flops": proof_flops, "time_ms": ms_per_step,
+                      "achieved": proof_flops / (ms_per_step * 1e-3) / 1e12, "unit": "TFLOP/s",
+                      "frac": proof_flops / (ms_per_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        if name == "graph" and os.path.exists(GRAPH_COUNTERS_FILE):
+            with open(GRAPH_COUNTERS_FILE) as fh:
+                gc = json.load(fh)
+            roof_proof["valu_issue_util"] = gc["SQ_INSTS_VALU"] * 4 / (ms_per_step * 1e-3 * VALU_ISSUE_HZ)
+            roof_proof["fp64_valu_util"] = gc["fp64_insts"] * 64 / (ms_per_step * 1e-3 * FP64_LANE_OPS)
+            roof_proof["counters"] = os.path.relpath(GRAPH_COUNTERS_FILE, ROOT)
+
     reference_grid = None if args.no_reference_grid else reference_grid_launches(cpl, abi)
 
     out = {
@@ -438,6 +525,7 @@ def main():
         "criteria_evaluations_per_s": criteria_evaluations / (ms_per_step * 1e-3),
         "unresolved": st_plain.unresolved,
         "roofline": roof,
+        "roofline_proof": roof_proof,
         "cpu_baseline": cpu,
         "cpu_parity": cpu_parity,
         "reference_grid_launch": reference_grid,

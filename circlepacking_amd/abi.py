@@ -152,6 +152,34 @@ class RefineStats(ctypes.Structure):
     def fp32_tried_total(self):
         return sum(self.fp32_tried[d][f] for d in range(MAX_ROUNDS) for f in range(6))
 
+    def unresolved_volume(self, cfg):
+        """Volume of the unresolved leaves as a fraction of the root box, per
+        family of cfg: a leaf of depth d covers 2^-(d b) of its root cell (b
+        axes bisected per split: 1 in split mode 0, the eligible axes in mode
+        1); unresolved leaves are CRITICAL at the maximum depth, NOFIT and TRAP
+        at any depth.  Independent of the split order, so runs that bisect
+        different axes compare by it."""
+        return self._leaf_volume(cfg, certified=False)
+
+    def certified_volume(self, cfg):
+        """Volume of the PROVEN, EMPTY and out-of-domain leaves, as unresolved_volume."""
+        return self._leaf_volume(cfg, certified=True)
+
+    def _leaf_volume(self, cfg, certified):
+        bits = 1 if cfg.split_mode == 0 else bin(cfg.split_axes).count("1")
+        out = {}
+        for f in cfg.families():
+            v = 0.0
+            for d in range(self.rounds):
+                fv = self.family_verdicts[d][f]
+                if certified:
+                    n = fv[V_PROVEN] + fv[V_EMPTY] + fv[V_OOD]
+                else:
+                    n = fv[V_NOFIT] + fv[V_TRAP] + (fv[V_CRITICAL] if d == cfg.max_depth else 0)
+                v += n * 2.0 ** (-d * bits)
+            out[FAMILY_NAMES[f]] = v / cfg.n_roots()
+        return out
+
     def as_dict(self):
         r = self.rounds
         return {
