@@ -130,6 +130,23 @@ def test_learned_split_leaves_fewer_unresolved_cells(cpl, trained):
     assert s.total_evaluations < plain.total_evaluations
 
 
+def test_learned_split_leaves_less_unresolved_volume(cpl, trained):
+    # on a grid the proposal never saw (trained on 24^4), at equal depth the
+    # learned split leaves less unresolved volume than the widest-axis rule;
+    # printed: the rule's volume one level deeper, for the record
+    base = _small()
+    plain, _ = cpl.refine_run(base)
+    s, _ = cpl.refine_run(abi.with_options(base, abi.OPT_LEARNED_SPLIT))
+    v_plain = sum(plain.unresolved_volume(base).values())
+    v_learned = sum(s.unresolved_volume(base).values())
+    deep = abi.cfg_4param(grid=(16, 16, 16, 16), max_depth=base.max_depth + 1)
+    d, _ = cpl.refine_run(deep)
+    v_deep = sum(d.unresolved_volume(deep).values())
+    print(f"unresolved volume: widest-axis {v_plain:.6g} (depth {base.max_depth}), {v_deep:.6g} (one deeper); "
+          f"learned split {v_learned:.6g}")
+    assert v_learned < 0.9 * v_plain, (v_learned, v_plain)
+
+
 def test_learned_rejected_with_persistent_form(cpl, trained):
     with pytest.raises(cpl.CplError):
         cpl.refine_run(abi.with_options(_small(), abi.OPT_PERSISTENT | abi.OPT_HIER_ROOTS | abi.OPT_LEARNED))
