@@ -56,23 +56,42 @@ __device__ __forceinline__ double sqrt_quarter(double t) {
 
 // Both ranges evaluate the one polynomial (in x^2 below 1/2, in t = (1 - |x|)/2
 // above), so a wave whose lanes fall in both ranges evaluates it once.
-__device__ __forceinline__ double asin_rn(double x) {
-  double ax = __builtin_fabs(x);
-  const bool small = ax <= 0.5;
-  double t = 0.5 - 0.5 * ax;  // exact for ax in [1/2, 1]
-  double z = small ? ax * ax : t;
-  double p = asin_poly_eval(z);
+struct AsinArg {
+  double ax, t, z;
+  bool small;
+};
+__device__ __forceinline__ AsinArg asin_arg(double x) {
+  AsinArg a;
+  a.ax = __builtin_fabs(x);
+  a.small = a.ax <= 0.5;
+  a.t = 0.5 - 0.5 * a.ax;  // exact for ax in [1/2, 1]
+  a.z = a.small ? a.ax * a.ax : a.t;
+  return a;
+}
+__device__ __forceinline__ double asin_finish(double x, const AsinArg& a, double p) {
   double r;
-  if (small) {
-    r = __builtin_fma(ax * z, p, ax);
+  if (a.small) {
+    r = __builtin_fma(a.ax * a.z, p, a.ax);
   } else {
     // asin(sqrt t) ~= s + w; the correction of s's own rounding error, (t - s^2) / 2s,
     // would cost an IEEE division for at most 0.4 ulp of the result
-    double s = sqrt_quarter(t);
-    double w = (s * t) * p;
+    double s = sqrt_quarter(a.t);
+    double w = (s * a.t) * p;
     r = 0x1.921fb54442d18p+0 - ((s + s) + (w + w - 0x1.1a62633145c07p-54));
   }
   return __builtin_copysign(r, x);
+}
+__device__ __forceinline__ double asin_rn(double x) {
+  const AsinArg a = asin_arg(x);
+  return asin_finish(x, a, asin_poly_eval(a.z));
+}
+// Two arcsines with their polynomials evaluated side by side (two independent
+// Horner chains the scheduler interleaves); each result is asin_rn's.
+__device__ __forceinline__ void asin_rn_pair(double x1, double x2, double& r1, double& r2) {
+  const AsinArg a1 = asin_arg(x1), a2 = asin_arg(x2);
+  const double p1 = asin_poly_eval(a1.z), p2 = asin_poly_eval(a2.z);
+  r1 = asin_finish(x1, a1, p1);
+  r2 = asin_finish(x2, a2, p2);
 }
 
 // nextafter(y, toward) for the arguments dasin uses (toward = +-2.0).
@@ -127,6 +146,23 @@ __device__ __forceinline__ double dasin_ru(double x) {
   if (y > 1.5707963267948968) y = 1.5707963267948968;
   if (x <= 0.0 && y > 0.0) y = 0.0;
   return y;
+}
+// Two widened arcsines (upper bound if UP1 / UP2) of independent arguments:
+// when every lane's arguments lie in [2^-1000, 1] both polynomials run side
+// by side; the same bits as dasin_rd / dasin_ru of each.
+template <bool UP1, bool UP2>
+__device__ __forceinline__ void dasin_pair(double x1, double x2, double& y1, double& y2) {
+  if (all_lanes(asin_unit_arg(x1) && asin_unit_arg(x2))) {
+    double a1, a2;
+    asin_rn_pair(x1, x2, a1, a2);
+    y1 = dfrom(UP1 ? dbits(a1) + 5 : dbits(a1) - 5);
+    y2 = dfrom(UP2 ? dbits(a2) + 5 : dbits(a2) - 5);
+    if (UP1 && y1 > 1.5707963267948968) y1 = 1.5707963267948968;
+    if (UP2 && y2 > 1.5707963267948968) y2 = 1.5707963267948968;
+    return;
+  }
+  y1 = UP1 ? dasin_ru(x1) : dasin_rd(x1);
+  y2 = UP2 ? dasin_ru(x2) : dasin_rd(x2);
 }
 
 }  // namespace cpl

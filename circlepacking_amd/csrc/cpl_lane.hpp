@@ -135,20 +135,24 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
           T lq, sq;
           bool empty;
           if (W == 1) {
-            lq = div_rd(lx.lb, lc.ub);  // lquot.lb
-            sq = div_ru(sx.ub, sc.lb);  // squot.ub, tightened to 1
-            if (T(1) < sq) sq = T(1);
+            div_pair2<false, true>(lx.lb, lc.ub, sx.ub, sc.lb, lq, sq);  // lquot.lb, squot.ub
+            if (T(1) < sq) sq = T(1);                                     // squot tightened to 1
             empty = lq > T(1) || rd_quot_above_one(sx.lb, sc.ub);
           } else {
-            lq = div_ru(lx.ub, lc.lb);  // lquot.ub, tightened to 1
-            if (T(1) < lq) lq = T(1);
-            sq = div_rd(sx.lb, sc.ub);  // squot.lb
+            div_pair2<true, false>(lx.ub, lc.lb, sx.lb, sc.ub, lq, sq);  // lquot.ub, squot.lb
+            if (T(1) < lq) lq = T(1);                                     // lquot tightened to 1
             empty = rd_quot_above_one(lx.lb, lc.ub) || sq > T(1);
           }
           if (empty) { trap = T_OBT_Q; return Angle<T>{result, trap}; }
           T rl = T(0), ru = T(0);
-          if (W == 1) rl = sub_rd(dasin_rd(lq), dasin_ru(sq));
-          else ru = sub_ru(dasin_ru(lq), dasin_rd(sq));
+          T al, as;  // the two widened arcsines, side by side
+          if (W == 1) {
+            dasin_pair<false, true>(lq, sq, al, as);
+            rl = sub_rd(al, as);
+          } else {
+            dasin_pair<true, false>(lq, sq, al, as);
+            ru = sub_ru(al, as);
+          }
           result = I(rl, ru);
           result.tighten_lb(T(0));
           done = true;
@@ -216,12 +220,17 @@ __device__ __attribute__((noinline)) Angle<T> compute_collision_angle_r(IVT<T> l
           if (lx.ub > lc.lb) { trap = T_ASIN; return Angle<T>{result, trap}; }
           if (sx.ub > sc.lb) { trap = T_ASIN; return Angle<T>{result, trap}; }
           I r(T(0), T(0));
+          T ql, qs, al, as;  // both quotients, then both widened arcsines, side by side
           if (W == 1) {
-            I beta(dasin_rd(div_rd(sx.lb, sc.ub)), T(0));
+            div_pair2<false, false>(lx.lb, lc.ub, sx.lb, sc.ub, ql, qs);
+            dasin_pair<false, false>(ql, qs, al, as);
+            I beta(as, T(0));
             beta.tighten_lb(T(0));
-            r.lb = add_rd(dasin_rd(div_rd(lx.lb, lc.ub)), beta.lb);
+            r.lb = add_rd(al, beta.lb);
           } else {
-            r.ub = add_ru(dasin_ru(div_ru(lx.ub, lc.lb)), dasin_ru(div_ru(sx.ub, sc.lb)));
+            div_pair2<true, true>(lx.ub, lc.lb, sx.ub, sc.lb, ql, qs);
+            dasin_pair<true, true>(ql, qs, al, as);
+            r.ub = add_ru(al, as);
           }
           if (result.empty()) result = r;
           else result.do_join(r);
@@ -382,15 +391,29 @@ __device__ __forceinline__ IVT<T> collision_angle_outer_outer(IVT<T> ra, IVT<T> 
   return a.v;
 }
 
-// collision_angles.cu:233-238
+// collision_angles.cu:233-238 (FP64: both quotients and both arcsines side by side)
 template <class T>
 __device__ __forceinline__ IVT<T> semicircle_angle_outer(IVT<T> r) {
-  return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(T(1), -r.lb))), dasin_ru(div_ru(r.ub, add_rd(T(1), -r.ub))));
+  if constexpr (std::is_same<T, double>::value) {
+    double ql, qu, yl, yu;
+    div_pair2<false, true>(r.lb, add_ru(1.0, -r.lb), r.ub, add_rd(1.0, -r.ub), ql, qu);
+    dasin_pair<false, true>(ql, qu, yl, yu);
+    return IV(yl, yu);
+  } else {
+    return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(T(1), -r.lb))), dasin_ru(div_ru(r.ub, add_rd(T(1), -r.ub))));
+  }
 }
 // collision_angles.cu:240-246
 template <class T>
 __device__ __forceinline__ IVT<T> semicircle_angle_inner(IVT<T> la, IVT<T> r) {
-  return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))), dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
+  if constexpr (std::is_same<T, double>::value) {
+    double ql, qu, yl, yu;
+    div_pair2<false, true>(r.lb, add_ru(la.ub, r.lb), r.ub, add_rd(la.lb, r.ub), ql, qu);
+    dasin_pair<false, true>(ql, qu, yl, yu);
+    return IV(yl, yu);
+  } else {
+    return IVT<T>(dasin_rd(div_rd(r.lb, add_ru(la.ub, r.lb))), dasin_ru(div_ru(r.ub, add_rd(la.lb, r.ub))));
+  }
 }
 
 // semicircle_angle_outer(r1), or `row` when it holds that angle for an r1 with

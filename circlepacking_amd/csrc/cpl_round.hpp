@@ -255,20 +255,47 @@ __device__ __forceinline__ double div_dir(double a, double b) {
 __device__ __forceinline__ double div_rd(double a, double b) { return div_dir<false>(a, b); }
 __device__ __forceinline__ double div_ru(double a, double b) { return div_dir<true>(a, b); }
 
-// [rd(al / bl), ru(au / bu)]: both corrections in one statement.
-__device__ __forceinline__ void div_pair(double al, double bl, double au, double bu, double& l, double& u) {
-  double ql = div_faithful(al, bl), qu = div_faithful(au, bu);
-  if (__builtin_expect(div_fast_ok(ql, bl) & div_fast_ok(qu, bu), 1)) {
-    double yl = div_y(bl), yu = div_y(bu), rl, ru;
-    asm("s_nop 1\n\t" CPL_MODE_RD "v_fma_f64 %2, -%4, %5, %6\n\t"
-        "v_fma_f64 %0, %2, %7, %4\n\t" CPL_MODE_RU "v_fma_f64 %3, -%8, %9, %10\n\t"
-        "v_fma_f64 %1, %3, %11, %8\n\t" CPL_MODE_RN
-        : "=&v"(l), "=&v"(u), "=&v"(rl), "=&v"(ru)
-        : "v"(ql), "v"(bl), "v"(al), "v"(yl), "v"(qu), "v"(bu), "v"(au), "v"(yu));
+// The correction statements of div_pair2: both remainders, then both fixes,
+// in one mode (SAME) or in two modes (M1 for the first, M2 for the second).
+#define CPL_DIV2_SAME(M)                                                                     \
+  asm("s_nop 1\n\t" M "v_fma_f64 %2, -%4, %5, %6\n\t"                                      \
+      "v_fma_f64 %3, -%8, %9, %10\n\tv_fma_f64 %0, %2, %7, %4\n\t"                          \
+      "v_fma_f64 %1, %3, %11, %8\n\t" CPL_MODE_RN                                            \
+      : "=&v"(q1), "=&v"(q2), "=&v"(r1), "=&v"(r2)                                          \
+      : "v"(f1), "v"(b1), "v"(a1), "v"(y1), "v"(f2), "v"(b2), "v"(a2), "v"(y2))
+#define CPL_DIV2_TWO(M1, M2)                                                                 \
+  asm("s_nop 1\n\t" M1 "v_fma_f64 %2, -%4, %5, %6\n\t"                                     \
+      "v_fma_f64 %0, %2, %7, %4\n\t" M2 "v_fma_f64 %3, -%8, %9, %10\n\t"                   \
+      "v_fma_f64 %1, %3, %11, %8\n\t" CPL_MODE_RN                                            \
+      : "=&v"(q1), "=&v"(q2), "=&v"(r1), "=&v"(r2)                                          \
+      : "v"(f1), "v"(b1), "v"(a1), "v"(y1), "v"(f2), "v"(b2), "v"(a2), "v"(y2))
+// Two independent directed quotients a1 / b1 (upward if UP1) and a2 / b2
+// (upward if UP2): the faithful quotients interleave, and both corrections run
+// in one statement (two mode switches fewer than two separate divisions).  The
+// same operations per quotient as div_dir, so the same bits.
+template <bool UP1, bool UP2>
+__device__ __forceinline__ void div_pair2(double a1, double b1, double a2, double b2, double& q1, double& q2) {
+  const double f1 = div_faithful(a1, b1), f2 = div_faithful(a2, b2);
+  if (__builtin_expect(div_fast_ok(f1, b1) & div_fast_ok(f2, b2), 1)) {
+    const double y1 = div_y(b1), y2 = div_y(b2);
+    double r1, r2;
+    if constexpr (UP1 && UP2) CPL_DIV2_SAME(CPL_MODE_RU);
+    else if constexpr (!UP1 && !UP2) CPL_DIV2_SAME(CPL_MODE_RD);
+    else if constexpr (UP1) CPL_DIV2_TWO(CPL_MODE_RU, CPL_MODE_RD);
+    else CPL_DIV2_TWO(CPL_MODE_RD, CPL_MODE_RU);
     return;
   }
-  l = div_fast_ok(ql, bl) ? div_fix_rd(ql, bl, al, div_y(bl)) : div_rare<false>(al, bl);
-  u = div_fast_ok(qu, bu) ? div_fix_ru(qu, bu, au, div_y(bu)) : div_rare<true>(au, bu);
+  q1 = div_fast_ok(f1, b1) ? (UP1 ? div_fix_ru(f1, b1, a1, div_y(b1)) : div_fix_rd(f1, b1, a1, div_y(b1)))
+                           : div_rare<UP1>(a1, b1);
+  q2 = div_fast_ok(f2, b2) ? (UP2 ? div_fix_ru(f2, b2, a2, div_y(b2)) : div_fix_rd(f2, b2, a2, div_y(b2)))
+                           : div_rare<UP2>(a2, b2);
+}
+#undef CPL_DIV2_SAME
+#undef CPL_DIV2_TWO
+
+// [rd(al / bl), ru(au / bu)]: both corrections in one statement.
+__device__ __forceinline__ void div_pair(double al, double bl, double au, double bu, double& l, double& u) {
+  div_pair2<false, true>(al, bl, au, bu, l, u);
 }
 
 // ---- square root ------------------------------------------------------------
