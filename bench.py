@@ -23,6 +23,8 @@ and reported separately (evaluations, unresolved leaves), not as `value`.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-tune] [--no-learned] [--forms plain,...]
   python bench.py --traffic-probe   # 3 x 256 MiB device copy + one plain run (HBM counter calibration)
+  python bench.py --graph-counters [graph|fp32|fp32+learned|learned-split] [--mlp FILE]
+                                    # five proofs of one batched form (for rocprofv3); FILE from --save-mlp
 """
 import argparse
 import json
@@ -190,6 +192,30 @@ def summarize_graph_pmc(dirs):
     return out
 
 
+def summarize_graph_pmc_by_kernel(dirs):
+    """Per-kernel totals of the same passes (template arguments dropped: the
+    instances of one kernel are summed), per proof."""
+    import csv
+    import glob
+
+    tot = {}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as fh:
+                for row in csv.DictReader(fh):
+                    name = row["Kernel_Name"].split("(")[0].split("<")[0].strip()
+                    if name.startswith("__amd_rocclr"):
+                        continue
+                    k = tot.setdefault(name, {})
+                    k[row["Counter_Name"]] = k.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    return {n: {c: v / GRAPH_RUNS for c, v in k.items()} for n, k in sorted(tot.items())}
+
+
+# batched forms that `bench.py --graph-counters FORM` runs (options beside the
+# graph form's graph + hierarchical roots + no digests)
+GRAPH_FORMS = ("graph", "fp32", "fp32+learned", "learned-split")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,14 +244,22 @@ def main():
                     help="rocprofv3 --pmc output directories (csv) of --traffic-probe runs: write the per-launch "
                          "counter averages of every kernel as JSON to --out, then exit (no GPU needed)")
     ap.add_argument("--out", default=None, help="output file of --summarize-pmc / --summarize-graph-pmc")
-    ap.add_argument("--graph-counters", action="store_true",
-                    help="run GRAPH_RUNS graph-form proofs (--graph-shape, default 64,2), then exit (for rocprofv3 --pmc)")
+    ap.add_argument("--graph-counters", nargs="?", const="graph", default=None, choices=GRAPH_FORMS,
+                    help="run GRAPH_RUNS proofs of one batched form (--graph-shape, default 64,2; the learned forms "
+                         "load --mlp), then exit (for rocprofv3)")
     ap.add_argument("--summarize-graph-pmc", nargs="+", metavar="DIR",
                     help="rocprofv3 --pmc output directories of --graph-counters runs: write the instruction totals "
                          "of one proof as JSON to --out, then exit (no GPU needed)")
+    ap.add_argument("--by-kernel", action="store_true",
+                    help="with --summarize-graph-pmc: the totals per kernel as well")
+    ap.add_argument("--save-mlp", default=None, metavar="FILE",
+                    help="write the proposals the benchmark trains to FILE (.npz, one array per family)")
+    ap.add_argument("--mlp", default=None, metavar="FILE", help="proposals for --graph-counters (from --save-mlp)")
     args = ap.parse_args()
     if args.summarize_graph_pmc:
         summary = summarize_graph_pmc(args.summarize_graph_pmc)
+        if args.by_kernel:
+            summary["by_kernel"] = summarize_graph_pmc_by_kernel(args.summarize_graph_pmc)
         text = json.dumps(summary, indent=1, sort_keys=True)
         if args.out:
             with open(args.out, "w") as fh:
@@ -262,11 +296,22 @@ def main():
         return
     if args.graph_counters:
         b, c = (int(x) for x in (args.graph_shape or "64,2").split(","))
-        cfg = abi.with_options(abi.cfg_4param(n=args.grid, max_depth=args.depth),
-                               abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST, block=b, cpt=c)
+        base = abi.cfg_4param(n=args.grid, max_depth=args.depth)
+        opts = abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST
+        extra = {"graph": 0, "fp32": abi.OPT_FP32, "fp32+learned": abi.OPT_FP32 | abi.OPT_LEARNED,
+                 "learned-split": abi.OPT_LEARNED_SPLIT}[args.graph_counters]
+        if extra & (abi.OPT_LEARNED | abi.OPT_LEARNED_SPLIT):
+            if not args.mlp:
+                sys.exit("bench.py: --graph-counters " + args.graph_counters + " needs --mlp FILE (from --save-mlp)")
+            with np.load(args.mlp) as z:
+                for f in base.families():
+                    cpl.mlp_install(f, z[abi.FAMILY_NAMES[f]])
+        cfg = abi.with_options(base, opts | extra, block=b, cpt=c,
+                               threshold=-0.5 if extra & abi.OPT_LEARNED else None)
         for _ in range(GRAPH_RUNS):
             cpl.refine_run(cfg, want_flags=False)
-        print(json.dumps({"graph_counters": True, "runs": GRAPH_RUNS, "block": b, "cells_per_thread": c}), flush=True)
+        print(json.dumps({"graph_counters": args.graph_counters, "runs": GRAPH_RUNS, "block": b,
+                          "cells_per_thread": c}), flush=True)
         return
     if args.reference_grid_only:
         print(json.dumps({"reference_grid_launch": reference_grid_launches(cpl, abi)}), flush=True)
@@ -338,6 +383,8 @@ def main():
         learned_info = {"train_seconds": time.perf_counter() - t0,
                         "records": {abi.FAMILY_NAMES[f]: m[1]["records"] for f, m in models.items()},
                         "final_loss": {abi.FAMILY_NAMES[f]: (m[1].get("loss") or [None])[-1] for f, m in models.items()}}
+        if args.save_mlp:
+            np.savez(args.save_mlp, **{abi.FAMILY_NAMES[f]: m[0] for f, m in models.items() if m[0] is not None})
         lg_cfg = abi.with_options(graph_cfg, G | H | ND | F32 | L, threshold=-0.5)
         ms, st = timed(lg_cfg)
         forms["graph+fp32+learned"] = {

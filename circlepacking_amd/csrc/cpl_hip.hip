@@ -257,10 +257,9 @@ __device__ __forceinline__ int widest_axis(const RefineParams& P, uint32_t meta)
 // operands already in registers: outputs 0..4 of each lane's cell.  It runs in a
 // kernel of its own (k_propose) ahead of each round, so that the evaluation
 // kernels keep their register budget.
-__device__ __forceinline__ void propose(const RefineParams& P, const MlpRegs& W, bool valid, const double* e,
-                                        float* y) {
+__device__ __forceinline__ void propose(const MlpBox& B, const MlpRegs& W, bool valid, const double* e, float* y) {
   float f[8];
-  if (valid) mlp_features(e, P.box_lb, P.box_ub, f);
+  if (valid) mlp_features(e, B, f);
   else
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] = 0.0f;
@@ -289,11 +288,15 @@ __device__ __forceinline__ int split_axis(const RefineParams& P, uint32_t meta, 
 // Each wave loads the network's operands into registers once and then walks
 // 64-cell groups with a wave-level grid stride (the operands are ~23 KB per
 // wave; reloading them per group dominated the kernel's time).
-__global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, uint64_t n, uint8_t* out) {
+// Two waves per SIMD (at most 256 registers): the MFMA results then go to
+// vector registers directly instead of accumulation registers read back one
+// by one.
+__global__ __launch_bounds__(256, 2) void k_propose(RefineParams P, Worklist in, uint64_t n, uint8_t* out) {
   uint64_t incl = 0;
   if (P.in_seg) n = seg_scan(P.in_seg, P.seg_cap, incl);
   MlpRegs W;
   mlp_load(P.mlp, W);
+  const MlpBox B = mlp_box(P.box_lb, P.box_ub);
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
   const uint64_t lane = threadIdx.x & 63;
   for (uint64_t grp = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); grp * 64 < n; grp += waves) {
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(256) void k_propose(RefineParams P, Worklist in, ui
       meta = in.meta(a);
     }
     float y[5];
-    propose(P, W, valid, e, y);
+    propose(B, W, valid, e, y);
     if (valid) out[i] = proposal_byte(P, y, meta);
   }
 }
@@ -759,7 +762,7 @@ __global__ __launch_bounds__(256) void k_record(RefineParams P, Worklist in, uin
       }
     }
     float f[8];
-    mlp_features(e, P.box_lb, P.box_ub, f);
+    mlp_features(e, mlp_box(P.box_lb, P.box_ub), f);
     const uint32_t pos = atomicAdd(S.count, 1u);
     if (pos < S.cap) {
 #pragma unroll
@@ -1173,6 +1176,9 @@ using cpl_host::DevBuf;
 using cpl_host::fail;
 using cpl_host::g_err;
 
+constexpr size_t kStageStats = (size_t)CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * sizeof(RoundStats);
+constexpr size_t kStageBytes = kStageStats + CPL_NUM_FAMILIES * sizeof(uint32_t);
+
 struct Context {
   bool ready = false;
   int device = 0;
@@ -1182,6 +1188,11 @@ struct Context {
   hipStream_t fam_stream[CPL_NUM_FAMILIES] = {};   // per-family branches (default priority)
   hipEvent_t fork_ev = nullptr, join_ev[CPL_NUM_FAMILIES] = {};
   hipEvent_t chain_end[CPL_NUM_FAMILIES] = {};  // graph form: end of each family chain (timed)
+  hipEvent_t run_begin = nullptr, run_end = nullptr;  // graph form: the replay's bounds (timed)
+  // pinned host copy of the summed statistics records and the families'
+  // overflow words, read back with one copy after a graph replay (allocated
+  // by the first refinement run)
+  unsigned char* stage = nullptr;
   DevBuf scratch_in, scratch_in2, scratch_out, cells, verdicts;
   DevBuf wl[2];           // ping-pong worklists; each holds nfam regions
   DevBuf la_table, root_flags, stats, stats_sum, counters;
@@ -1213,6 +1224,8 @@ int ensure_init() {
     HIP_TRY(hipEventCreate(&g.chain_end[f]));
   }
   HIP_TRY(hipEventCreateWithFlags(&g.fork_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreate(&g.run_begin));
+  HIP_TRY(hipEventCreate(&g.run_end));
   // the device address of this device's copy of the no-allocation scratch
   void* sp = nullptr;
   HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
@@ -1348,6 +1361,10 @@ int cpl_shutdown(void) {
     g.fam_stream[f] = nullptr;
   }
   (void)hipEventDestroy(g.fork_ev);
+  (void)hipEventDestroy(g.run_begin);
+  (void)hipEventDestroy(g.run_end);
+  (void)hipHostFree(g.stage);
+  g.stage = nullptr;
   g.ready = false;
   return CPL_OK;
 }
@@ -1678,16 +1695,37 @@ template <int F> struct LaunchTriples {
 // graph-form run (sizes the next capture's depth-0 round).
 uint64_t g_live_items[CPL_NUM_FAMILIES];
 
+int ensure_stage() {
+  if (!g.stage) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.stage), kStageBytes, hipHostMallocDefault));
+  return CPL_OK;
+}
+
+// Sums the statistics copies on the device and queues their copy into the
+// pinned stage (read after the stream has been synchronised).
+int collect_enqueue() {
+  const int words = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * (int)(sizeof(RoundStats) / 8);
+  hipLaunchKernelGGL(k_reduce_stats, dim3((words + 255) / 256), dim3(256), 0, g.stream, g.stats.as<RoundStats>(),
+                     g.stats_sum.as<RoundStats>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(g.stage, g.stats_sum.p, kStageStats, hipMemcpyDeviceToHost, g.stream));
+  return CPL_OK;
+}
+
+int collect_staged(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags);
+
 int collect(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
+  int rc = ensure_stage();
+  if (rc) return rc;
+  rc = collect_enqueue();
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g.stream));
+  return collect_staged(pl, stats, root_flags);
+}
+
+// The rest of collect() once the stage holds the summed records.
+int collect_staged(const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
   std::vector<RoundStats> rs(CPL_MAX_ROUNDS * CPL_NUM_FAMILIES);
-  {
-    const int words = CPL_MAX_ROUNDS * CPL_NUM_FAMILIES * (int)(sizeof(RoundStats) / 8);
-    hipLaunchKernelGGL(k_reduce_stats, dim3((words + 255) / 256), dim3(256), 0, g.stream, g.stats.as<RoundStats>(),
-                       g.stats_sum.as<RoundStats>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(rs.data(), g.stats_sum.p, rs.size() * sizeof(RoundStats), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-  }
+  std::memcpy(rs.data(), g.stage, kStageStats);
   if (root_flags) HIP_TRY(hipMemcpy(root_flags, g.root_flags.p, pl.n_roots * pl.nf, hipMemcpyDeviceToHost));
   double seconds = stats->seconds, eval_seconds = stats->eval_seconds;
   uint64_t peak = stats->peak_items;
@@ -1863,6 +1901,7 @@ struct GraphCache {
 };
 GraphCache g_graph;
 
+
 unsigned grid_tuned(uint64_t expect, const Plan& pl) {
   uint64_t per_block = (uint64_t)pl.block * pl.cpt;
   uint64_t want = (std::max<uint64_t>(expect, 1) + per_block - 1) / per_block;
@@ -1966,9 +2005,9 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
     g_graph.flags = pl.flags;
     g_graph.valid = true;
   }
-  hipEvent_t ev0, ev1;
-  HIP_TRY(hipEventCreate(&ev0));
-  HIP_TRY(hipEventCreate(&ev1));
+  int rc = ensure_stage();
+  if (rc) return rc;
+  hipEvent_t ev0 = g.run_begin, ev1 = g.run_end;
   HIP_TRY(hipEventRecord(ev0, g.stream));
   const bool prologue = g_graph.exec[0] != nullptr;
   if (prologue) {
@@ -2009,6 +2048,11 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   }
   for (int s = 1; s < pl.nf; ++s) HIP_TRY(hipStreamWaitEvent(g.stream, g.join_ev[s], 0));
   HIP_TRY(hipEventRecord(ev1, g.stream));
+  // statistics and overflow words come back with the replay's one synchronisation
+  rc = collect_enqueue();
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(g.stage + kStageStats, overflow_word(0), pl.nf * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                         g.stream));
   HIP_TRY(hipStreamSynchronize(g.stream));
   for (int s = 0; s < pl.nf; ++s) {
     float m = 0.f;
@@ -2016,7 +2060,7 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
     g_graph.chain_ms[s] = m;
   }
   uint32_t ovfw[CPL_NUM_FAMILIES] = {};
-  HIP_TRY(hipMemcpy(ovfw, overflow_word(0), pl.nf * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::memcpy(ovfw, g.stage + kStageStats, pl.nf * sizeof(uint32_t));
   uint32_t ovf = 0;
   for (int s = 0; s < pl.nf; ++s) ovf |= ovfw[s];
   std::memset(stats, 0, sizeof(*stats));
@@ -2024,13 +2068,11 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
   HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
   stats->seconds = ms * 1e-3;
   stats->eval_seconds = stats->seconds;
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   if (ovf) {
     g_graph.reset();
     return fail(CPL_ERR_CAPACITY, "worklist capacity exceeded; raise CPL_WORKLIST_CAP");
   }
-  int rc = collect(pl, stats, root_flags);
+  rc = collect_staged(pl, stats, root_flags);
   if (rc) return rc;
   // Record the round sizes of this replay; if they differ from the ones the
   // graph's grids were sized for, re-capture on the next call.

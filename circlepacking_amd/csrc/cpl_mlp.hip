@@ -248,8 +248,8 @@ void cpl_mlp_pack(const float* w, void* out_packed) {
     float b1[4][64][4];
     uint16_t a2[4][2][64][8];
     float b2[4][64][4];
-    uint16_t a3[2][64][8];
-    float b3[64][4];
+    uint16_t a3[4][2][64][8];
+    float b3[4][64][4];
   };
   static_assert(sizeof(Host) == sizeof(MlpPacked), "packed layout");
   Host h;
@@ -258,8 +258,8 @@ void cpl_mlp_pack(const float* w, void* out_packed) {
     const int g = lane / 16, m = lane % 16;
     for (int t = 0; t < 4; ++t) {
       for (int i = 0; i < 8; ++i) {
-        const int k = 8 * g + i;  // layer-1 contraction index: the feature
-        h.a1[t][lane][i] = bf16_bits(k < MLP_IN ? w[MLP_W1 + k * MLP_H + 16 * t + m] : 0.0f);
+        // layer 1 contracts over feature i in every K slice g (one slice per pass)
+        h.a1[t][lane][i] = bf16_bits(w[MLP_W1 + i * MLP_H + 16 * t + m]);
         for (int s = 0; s < 2; ++s) h.a2[t][s][lane][i] = bf16_bits(w[MLP_W2 + mlp_unit(s, g, i) * MLP_H + 16 * t + m]);
       }
       for (int r = 0; r < 4; ++r) {
@@ -267,10 +267,22 @@ void cpl_mlp_pack(const float* w, void* out_packed) {
         h.b2[t][lane][r] = w[MLP_B2 + 16 * t + 4 * g + r];
       }
     }
-    for (int s = 0; s < 2; ++s)
-      for (int i = 0; i < 8; ++i)
-        h.a3[s][lane][i] = bf16_bits(m < MLP_OUT ? w[MLP_W3 + mlp_unit(s, g, i) * MLP_OUT + m] : 0.0f);
-    for (int r = 0; r < 4; ++r) h.b3[lane][r] = (4 * g + r < MLP_OUT) ? w[MLP_B3 + 4 * g + r] : 0.0f;
+    // layer 3, pass j: row 4 j + r is output r, row 4 ((j + 1) mod 4) + r output 4 + r, other rows zero
+    auto out_of_row = [](int j, int row) {
+      if (row / 4 == j) return row % 4;
+      if (row / 4 == ((j + 1) & 3)) return 4 + row % 4;
+      return -1;
+    };
+    for (int j = 0; j < 4; ++j) {
+      const int om = out_of_row(j, m);
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < 8; ++i)
+          h.a3[j][s][lane][i] = bf16_bits(om >= 0 ? w[MLP_W3 + mlp_unit(s, g, i) * MLP_OUT + om] : 0.0f);
+      for (int r = 0; r < 4; ++r) {
+        const int orow = out_of_row(j, 4 * g + r);
+        h.b3[j][lane][r] = orow >= 0 ? w[MLP_B3 + orow] : 0.0f;
+      }
+    }
   }
   std::memcpy(out_packed, &h, sizeof(h));
 }
