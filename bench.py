@@ -252,6 +252,9 @@ def main():
                          "of one proof as JSON to --out, then exit (no GPU needed)")
     ap.add_argument("--by-kernel", action="store_true",
                     help="with --summarize-graph-pmc: the totals per kernel as well")
+    ap.add_argument("--key", default=None,
+                    help="with --summarize-graph-pmc: store the totals under this key of the --out file, keeping "
+                         "the file's other keys")
     ap.add_argument("--save-mlp", default=None, metavar="FILE",
                     help="write the proposals the benchmark trains to FILE (.npz, one array per family)")
     ap.add_argument("--mlp", default=None, metavar="FILE", help="proposals for --graph-counters (from --save-mlp)")
@@ -260,6 +263,13 @@ def main():
         summary = summarize_graph_pmc(args.summarize_graph_pmc)
         if args.by_kernel:
             summary["by_kernel"] = summarize_graph_pmc_by_kernel(args.summarize_graph_pmc)
+        if args.key:
+            merged = {}
+            if args.out and os.path.exists(args.out):
+                with open(args.out) as fh:
+                    merged = json.load(fh)
+            merged[args.key] = summary
+            summary = merged
         text = json.dumps(summary, indent=1, sort_keys=True)
         if args.out:
             with open(args.out, "w") as fh:
@@ -397,22 +407,39 @@ def main():
         def volume(stats, cfg):  # unresolved volume, fraction of the box summed over the families
             return sum(stats.unresolved_volume(cfg).values())
 
-        # the same measure one level deeper without the proposal: the depth the
-        # widest-axis rule needs to leave no more unresolved volume
-        deep = abi.cfg_4param(n=args.grid, max_depth=args.depth + 1)
-        deep_cfg = abi.with_options(deep, G | H | ND, block=graph_cfg.block, cpt=graph_cfg.cells_per_thread)
-        ms_deep, st_deep = timed(deep_cfg)
+        # the widest-axis rule one and two levels deeper (plain and graph forms):
+        # the depth it needs to leave no more unresolved volume than the learned
+        # split leaves at the benchmark's depth, and the time that takes
+        target = volume(st, base)
+        deeper, same_volume = [], None
+        for d in (args.depth + 1, args.depth + 2):
+            deep = abi.cfg_4param(n=args.grid, max_depth=d)
+            rows = {}
+            for fname, dcfg in (("plain", abi.with_options(deep, ND)),
+                                ("graph", abi.with_options(deep, G | H | ND, block=graph_cfg.block,
+                                                           cpt=graph_cfg.cells_per_thread))):
+                ms_d, st_d = timed(dcfg)
+                rows[fname] = {"max_depth": d, "form": fname, "ms": ms_d, "evaluations": st_d.total_evaluations,
+                               "unresolved_volume": volume(st_d, deep)}
+            deeper += [rows["plain"], rows["graph"]]
+            reached = rows["plain"]["unresolved_volume"] <= target
+            same_volume = {"reached": reached, "max_depth": d, "plain_ms": rows["plain"]["ms"],
+                           "graph_ms": rows["graph"]["ms"], "speedup_vs_plain": rows["plain"]["ms"] / ms,
+                           "speedup_vs_graph": rows["graph"]["ms"] / ms}
+            if reached:
+                break
         learned_info["learned_split"] = {
             "ms": ms, "evaluations": st.total_evaluations, "certified": st.certified, "unresolved": st.unresolved,
-            "unresolved_volume": volume(st, base),
+            "unresolved_volume": target,
             "plain_evaluations": st_plain.total_evaluations, "plain_unresolved": st_plain.unresolved,
             "plain_unresolved_volume": volume(st_plain, base),
-            "plain_one_level_deeper": {"max_depth": args.depth + 1, "ms": ms_deep,
-                                       "evaluations": st_deep.total_evaluations,
-                                       "unresolved_volume": volume(st_deep, deep)},
+            "widest_axis_deeper": deeper,
+            "widest_axis_to_same_volume": same_volume,
             "note": "changes the set of evaluated cells (another bisection order); not an identical-verdict form. "
                     "unresolved_volume: volume of the leaves left unresolved, as a fraction of the box, summed over "
-                    "the families",
+                    "the families; widest_axis_to_same_volume: the first depth at which the widest-axis rule leaves "
+                    "no more unresolved volume, and its time against the learned split's; with reached false, the "
+                    "deepest level tried, whose speedups are lower bounds",
         }
         for f in range(6):
             cpl.mlp_install(f, None)
