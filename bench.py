@@ -219,8 +219,8 @@ GRAPH_FORMS = ("graph", "fp32", "fp32+learned", "learned-split")
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--variant", default="tuned", choices=["plain", "tuned"],
                     help="which form `value` reports; every form is timed and checked for identical results")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (256 threads, 1 cell/thread)")

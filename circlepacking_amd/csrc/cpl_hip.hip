@@ -323,10 +323,9 @@ __global__ __launch_bounds__(256, 2) void k_propose(RefineParams P, Worklist in,
 // P.prefilter_depths, gated by the proposal byte).  The proposal byte itself
 // (P.prop, written by k_propose) is read by every refinement kernel.
 // WAVES: the second launch bound, waves per SIMD the registers are budgeted for:
-// 3 (<= 168 VGPRs) for the plain form; 4 (<= 128 VGPRs, spills) for the graph
-// form's 256-thread kernels, where the four families' rounds share the GPU and
-// the extra wave measured 1-2 % faster per replay (the plain form's rounds ran
-// 2 % slower with it).
+// 3 (<= 168 VGPRs) in every form.  With 4 (<= 128 VGPRs) the families' kernels
+// spill 21 to 81 VGPRs per lane and the graph form's proof took 11 % longer;
+// with 2 both the graph and the plain form were slower.
 template <int F, bool ROOT, int LB = 256, bool EXT = false, int WAVES = 3>
 __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in, uint64_t n) {
   uint64_t incl = 0;
@@ -1282,14 +1281,8 @@ template <int F> struct LaunchRoundB {
   static void go(unsigned grid, int block, const RefineParams& P, const Worklist& in, uint64_t n) {
     hipLaunchKernelGGL((k_round<F, R, LB, EXT, WAVES>), dim3(grid), dim3(block), 0, launch_stream(), P, in, n);
   }
-  // graph: the graph form's budget of four waves per SIMD for 256-thread FP64 kernels
   static void run(bool root, bool p32, unsigned grid, int block, const RefineParams& P, const Worklist& in,
-                  uint64_t n, bool graph = false) {
-    if (graph && block <= 256 && !p32) {
-      if (root) go<true, 256, false, 4>(grid, block, P, in, n);
-      else go<false, 256, false, 4>(grid, block, P, in, n);
-      return;
-    }
+                  uint64_t n) {
     const int k = (root ? 4 : 0) | (block > 256 ? 2 : 0) | (p32 ? 1 : 0);
     switch (k) {
       case 0: go<false, 256, false>(grid, block, P, in, n); break;
@@ -1952,11 +1945,11 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
       dispatch_family<LaunchTriples>(pl.fams[s], grid_tuned(pl.n_triples, pl), pl.block, T);
       P.in_seg = counter(2, s);
       dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(live_of(), pl), pl.block, P,
-                                    region(g.wl[0], pl.cap, s), (uint64_t)0, true);
+                                    region(g.wl[0], pl.cap, s), (uint64_t)0);
     } else {
       Worklist none{};
       dispatch_family<LaunchRoundB>(pl.fams[s], true, pl.ext, grid_tuned(pl.n_roots, pl), pl.block, P, none,
-                                    pl.n_roots, true);
+                                    pl.n_roots);
     }
   } else {
     // round d reads buffer 1 - d % 2 ... rounds alternate buffers 1 -> 0 -> 1
@@ -1967,7 +1960,7 @@ void enqueue_round(const cpl_refine_cfg* cfg, const Plan& pl, int s, int d, cons
     P.zero_seg = rot_counter(d + 2, s);
     launch_propose(pl, s, P, region(g.wl[cur], pl.cap, s), 0, exp_of(d));
     dispatch_family<LaunchRoundB>(pl.fams[s], false, pl.ext, grid_tuned(exp_of(d), pl), pl.block, P,
-                                  region(g.wl[cur], pl.cap, s), (uint64_t)0, true);
+                                  region(g.wl[cur], pl.cap, s), (uint64_t)0);
   }
   g_launch_stream = nullptr;
 }
