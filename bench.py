@@ -18,6 +18,9 @@ Forms timed (each checked for identical results against the plain path):
   +learned    graph + FP32 prefilter gated by the learned proposal (MLP
               trained on the GPU from every record of the same 64^4 run,
               about 9.6e6, one per criteria evaluation)
+  +learned order  graph with the two-check families' checks in the order the
+              proposal predicts (same verdict kinds; failing-criterion details
+              may differ), reported beside the identical forms
 The learned split-axis policy changes the set of evaluated cells; it is timed
 and reported separately (evaluations, unresolved leaves), not as `value`.
 
@@ -213,7 +216,7 @@ def summarize_graph_pmc_by_kernel(dirs):
 
 # batched forms that `bench.py --graph-counters FORM` runs (options beside the
 # graph form's graph + hierarchical roots + no digests)
-GRAPH_FORMS = ("graph", "fp32", "fp32+learned", "learned-split")
+GRAPH_FORMS = ("graph", "fp32", "fp32+learned", "learned-split", "learned-order")
 
 
 def main():
@@ -309,8 +312,8 @@ def main():
         base = abi.cfg_4param(n=args.grid, max_depth=args.depth)
         opts = abi.OPT_GRAPH | abi.OPT_HIER_ROOTS | abi.OPT_NO_DIGEST
         extra = {"graph": 0, "fp32": abi.OPT_FP32, "fp32+learned": abi.OPT_FP32 | abi.OPT_LEARNED,
-                 "learned-split": abi.OPT_LEARNED_SPLIT}[args.graph_counters]
-        if extra & (abi.OPT_LEARNED | abi.OPT_LEARNED_SPLIT):
+                 "learned-split": abi.OPT_LEARNED_SPLIT, "learned-order": abi.OPT_LEARNED_ORDER}[args.graph_counters]
+        if extra & (abi.OPT_LEARNED | abi.OPT_LEARNED_SPLIT | abi.OPT_LEARNED_ORDER):
             if not args.mlp:
                 sys.exit("bench.py: --graph-counters " + args.graph_counters + " needs --mlp FILE (from --save-mlp)")
             with np.load(args.mlp) as z:
@@ -328,8 +331,8 @@ def main():
         return
     want = set(args.forms.split(","))
 
-    G, H, PER, F32, L, LS, ND = (abi.OPT_GRAPH, abi.OPT_HIER_ROOTS, abi.OPT_PERSISTENT, abi.OPT_FP32,
-                                 abi.OPT_LEARNED, abi.OPT_LEARNED_SPLIT, abi.OPT_NO_DIGEST)
+    G, H, PER, F32, L, LS, LO, ND = (abi.OPT_GRAPH, abi.OPT_HIER_ROOTS, abi.OPT_PERSISTENT, abi.OPT_FP32,
+                                     abi.OPT_LEARNED, abi.OPT_LEARNED_SPLIT, abi.OPT_LEARNED_ORDER, abi.OPT_NO_DIGEST)
     base = abi.cfg_4param(n=args.grid, max_depth=args.depth)
     plain_ref = cpl.refine_run(abi.with_options(base, 0))  # digests + flags of the plain path
 
@@ -400,6 +403,20 @@ def main():
         forms["graph+fp32+learned"] = {
             "ms": ms, "identical": identical(lg_cfg), "fp32_tried": st.fp32_tried_total(),
             "fp32_settled": st.fp32_settled_total(), "gate_threshold_logit": -0.5,
+        }
+        # learned check order: verdict kinds, counts, item digests and root flags
+        # are the plain path's; the failing-criterion details may differ
+        lo_cfg = abi.with_options(graph_cfg, G | H | ND | LO)
+        ms, st = timed(lo_cfg)
+        lo_s, lo_f = cpl.refine_run(abi.with_options(lo_cfg, G | H | LO))
+        dk, dp = lo_s.as_dict(), plain_ref[0].as_dict()
+        kind_keys = ("rounds", "evaluations", "verdicts", "digest", "family_verdicts", "total_evaluations",
+                     "certified", "unresolved")
+        forms["graph+learned order"] = {
+            "ms": ms, "identical": False,
+            "identical_verdict_kinds": all(dk[k] == dp[k] for k in kind_keys) and bool(np.array_equal(lo_f, plain_ref[1])),
+            "identical_failing_criteria": dk["verdict_digest"] == dp["verdict_digest"],
+            "note": "two-check families run first the check the proposals of a wave predict to fail first",
         }
         ls_cfg = abi.with_options(graph_cfg, G | H | ND | LS)
         ms, st = timed(ls_cfg)
@@ -583,7 +600,10 @@ def main():
         "forms": forms,
         "tuned_form": tuned_name,
         "tuned_speedup": forms["plain"]["ms"] / forms[tuned_name]["ms"],
-        "identical_verdicts": all(v["identical"] for v in forms.values()),
+        # every form that keeps the plain path's verdicts does (the learned check
+        # order is checked on verdict kinds instead)
+        "identical_verdicts": all(v["identical"] if "identical_verdict_kinds" not in v else v["identical_verdict_kinds"]
+                                  for v in forms.values()),
         "autotune_graph": tune_graph,
         "autotune_persistent": tune_persist,
         "learned": learned_info,

@@ -250,9 +250,9 @@ def mlp_grad(features, targets, params, want_outputs=False):
 
 
 def mlp_infer(features, params):
-    """Outputs 0..4 (n, 5) of the in-kernel BF16 MFMA inference."""
+    """Outputs 0..5 (n, 6) of the in-kernel BF16 MFMA inference."""
     x, w = _f32(features, 8), _f32(params)
-    out = np.zeros((x.shape[0], 5), dtype=np.float32)
+    out = np.zeros((x.shape[0], 6), dtype=np.float32)
     _check(lib().cpl_mlp_infer(_fptr(x), x.shape[0], _fptr(w), _fptr(out)))
     return out
 

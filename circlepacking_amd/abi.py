@@ -15,6 +15,7 @@ ERR_NAMES = {-1: "HIP", -2: "ARG", -3: "TRAP", -4: "NOFIT", -5: "CAPACITY", -6: 
 MAX_ROUNDS = 16
 
 OPT_GRAPH, OPT_PERSISTENT, OPT_HIER_ROOTS, OPT_FP32, OPT_LEARNED, OPT_LEARNED_SPLIT, OPT_NO_DIGEST = 1, 2, 4, 8, 16, 32, 256
+OPT_LEARNED_ORDER = 64
 
 MLP_INPUTS, MLP_TARGETS, MLP_NPARAM = 8, 8, 5256
 MLP_H, MLP_OUT = 64, 8

@@ -254,7 +254,7 @@ __device__ __forceinline__ int widest_axis(const RefineParams& P, uint32_t meta)
 }
 
 // Learned proposal of one wave (all 64 lanes call it) with the network's
-// operands already in registers: outputs 0..4 of each lane's cell.  It runs in a
+// operands already in registers: outputs 0..5 of each lane's cell.  It runs in a
 // kernel of its own (k_propose) ahead of each round, so that the evaluation
 // kernels keep their register budget.
 __device__ __forceinline__ void propose(const MlpBox& B, const MlpRegs& W, bool valid, const double* e, float* y) {
@@ -268,14 +268,15 @@ __device__ __forceinline__ void propose(const MlpBox& B, const MlpRegs& W, bool 
 
 // Proposal byte of a cell: bit 0 gate (run the FP32 prefilter), bits 1-2 the
 // split axis (the eligible axis of highest predicted score y[1 + a]; the
-// widest axis wins ties and NaN scores), bit 7 set.
+// widest axis wins ties and NaN scores), bit 3 the second check predicted to
+// fail first (y[5] > 0), bit 7 set.
 __device__ __forceinline__ uint8_t proposal_byte(const RefineParams& P, const float* y, uint32_t meta) {
   int axis = widest_axis(P, meta);
   float ls = y[1 + axis];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
     if ((P.split_axes & (1u << a)) && y[1 + a] > ls) { axis = a; ls = y[1 + a]; }
-  return (uint8_t)(0x80u | (uint32_t)(axis << 1) | (y[0] > P.mlp_threshold ? 1u : 0u));
+  return (uint8_t)(0x80u | (uint32_t)(axis << 1) | (y[0] > P.mlp_threshold ? 1u : 0u) | (y[5] > 0.0f ? 8u : 0u));
 }
 // Without a proposal: gate open, widest axis.
 constexpr uint8_t kNoProposal = 0x01;
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(256, 2) void k_propose(RefineParams P, Worklist in,
       for (int k = 0; k < 8; ++k) e[k] = in.e(k, a);
       meta = in.meta(a);
     }
-    float y[5];
+    float y[6];
     propose(B, W, valid, e, y);
     if (valid) out[i] = proposal_byte(P, y, meta);
   }
@@ -382,13 +383,20 @@ __global__ __launch_bounds__(LB, WAVES) void k_round(RefineParams P, Worklist in
     }
     uint8_t pb = kNoProposal;
     if (!ROOT && P.prop && valid) pb = P.prop[i];
+    // learned check order: the wave runs the second check first when most of
+    // its cells' proposals predict that check to fail first
+    bool second_first = false;
+    if (!ROOT && P.learned_order) {
+      const int voters = __popcll(__ballot(valid && (pb & 0x80)));
+      second_first = 2 * __popcll(__ballot(valid && (pb & 0x88) == 0x88)) > voters;
+    }
     if (valid) {
       int depth = (int)(meta >> 16);
       double lbrsp = 0.0;
       if (P.clip) lbrsp = P.la_table[3 * (la_level_offset(P.grid[0], axis_splits(meta, 0)) + la_idx) + 2];
       const bool pre = EXT && ((P.prefilter_depths >> depth) & 1u) && (pb & 1);
       vb = eval_synthetic<F, EXT>(IV(e[0], e[1]), IV(e[2], e[3]), IV(e[4], e[5]), IV(e[6], e[7]), P.clip != 0, lbrsp,
-                                  pre, &hit, &tried);
+                                  pre, &hit, &tried, second_first);
       int v = vb & 7;
       if (P.digest) {
         h = item_hash(e, root, F, meta);
@@ -1569,7 +1577,7 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.seg_cap = (uint32_t)(pl.cap / kSegs);
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
-  pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT)) != 0;
+  pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT | CPL_OPT_LEARNED_ORDER)) != 0;
   pl.ext = (cfg->options & CPL_OPT_FP32) != 0;
   pl.prefilter_depths = !(cfg->options & CPL_OPT_FP32) ? 0u : cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
   if (pl.learned)
@@ -1648,6 +1656,7 @@ RefineParams base_params(const cpl_refine_cfg* cfg, const Plan& pl) {
   P.prefilter_depths = pl.prefilter_depths;
   P.mlp_threshold = cfg->learned_threshold;
   P.learned_split = (cfg->options & CPL_OPT_LEARNED_SPLIT) ? 1 : 0;
+  P.learned_order = (cfg->options & CPL_OPT_LEARNED_ORDER) ? 1 : 0;
   return P;
 }
 
@@ -2205,7 +2214,7 @@ int cpl_refine_run(const cpl_refine_cfg* cfg, cpl_refine_stats* stats, uint8_t* 
   if ((rc = make_plan(cfg, root_flags != nullptr, pl))) return rc;
   if ((rc = ensure_buffers(pl))) return rc;
   const uint32_t known = CPL_OPT_GRAPH | CPL_OPT_PERSISTENT | CPL_OPT_HIER_ROOTS | CPL_OPT_FP32 | CPL_OPT_LEARNED |
-                         CPL_OPT_LEARNED_SPLIT | CPL_OPT_NO_DIGEST;
+                         CPL_OPT_LEARNED_SPLIT | CPL_OPT_LEARNED_ORDER | CPL_OPT_NO_DIGEST;
   if (cfg->options & ~known) return fail(CPL_ERR_ARG, "unsupported option bits");
   if ((cfg->options & CPL_OPT_PERSISTENT) && pl.learned)
     return fail(CPL_ERR_ARG, "the learned proposal runs with the round-based forms (plain or CPL_OPT_GRAPH)");

@@ -226,6 +226,7 @@ struct RefineParams {
   const MlpPacked* mlp;      // extended kernels: learned proposal of this family, or null
   float mlp_threshold;       // gate: the prefilter runs when output 0 exceeds this logit
   int32_t learned_split;     // 1: mode-0 splits take the proposal's best-scored eligible axis
+  int32_t learned_order;     // 1: a wave runs first the check its proposals predict to fail first
   const uint8_t* prop;       // extended round kernels: proposal byte per input item (k_propose), or null
   unsigned long long* zero_seg;  // segment counts cleared by the round kernel (the next round's output), or null
 };

@@ -663,8 +663,26 @@ __device__ __forceinline__ bool handle_r1_outer_3d(IVT<T> la, IVT<T> r1, IVT<T> 
 // call would save and restore ~40 callee-saved VGPRs per lane through scratch
 // (measured: HBM traffic of the round kernel x1.8 for no gain in time).
 // a1_row: semicircle_angle_outer(r1) when the caller has it (null otherwise).
+// Two-check families (middle, end 2-disk, end 3-disk): the checks in the
+// reference's order, or with second_first (wave-uniform; the learned order)
+// the second check first.  The result names the first failing check in the
+// order evaluated, so it differs from the reference's only where both fail.
+template <class C1, class C2>
+__device__ __forceinline__ int two_checks(bool second_first, C1 check1, C2 check2) {
+  int trap = 0;
+#pragma unroll 1
+  for (int k = 0; k < 2; ++k) {
+    const bool second = (k == 0) == second_first;
+    const bool ok = second ? check2(trap) : check1(trap);
+    if (trap) return 16 + trap;
+    if (!ok) return second ? 2 : 1;
+  }
+  return 0;
+}
+
 template <int F, class T>
-__device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, const IVT<T>* a1_row = nullptr) {
+__device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> r3, const IVT<T>* a1_row = nullptr,
+                                        bool second_first = false) {
   int trap = 0;
   if constexpr (F == F_START) {  // start_configuration.cu:142-162
     IVT<T> a12 = collision_angle_outer_inner(la, r1, r2, trap);
@@ -686,6 +704,10 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
     if (trap) return 16 + trap;
     return ok ? 0 : 1;
   } else if constexpr (F == F_MIDDLE) {
+    if (second_first)
+      return two_checks(
+          true, [&](int& tr) { return handle_case_1_1(la, r1, r2, r3, tr); },
+          [&](int& tr) { return handle_case_2_1(la, r1, r2, r3, tr); });
     bool ok = handle_case_1_1(la, r1, r2, r3, trap);
     if (trap) return 16 + trap;
     if (!ok) return 1;
@@ -693,6 +715,10 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
     if (trap) return 16 + trap;
     return ok ? 0 : 2;
   } else if constexpr (F == F_END_2D) {
+    if (second_first)
+      return two_checks(
+          true, [&](int& tr) { return handle_inner_outer_2d(la, r1, r2, tr); },
+          [&](int& tr) { return handle_outer_inner_2d(la, r1, r2, tr, a1_row); });
     bool ok = handle_inner_outer_2d(la, r1, r2, trap);
     if (trap) return 16 + trap;
     if (!ok) return 1;
@@ -700,6 +726,10 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
     if (trap) return 16 + trap;
     return ok ? 0 : 2;
   } else {
+    if (second_first)
+      return two_checks(
+          true, [&](int& tr) { return handle_r1_inner_3d(la, r1, r2, r3, tr); },
+          [&](int& tr) { return handle_r1_outer_3d(la, r1, r2, r3, tr); });
     bool ok = handle_r1_inner_3d(la, r1, r2, r3, trap);
     if (trap) return 16 + trap;
     if (!ok) return 1;
@@ -721,7 +751,7 @@ __device__ __forceinline__ int criteria(IVT<T> la, IVT<T> r1, IVT<T> r2, IVT<T> 
 template <int F, bool PREFILTER = false>
 __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3, bool pre = true,
                                                     bool* fp32_hit = nullptr, bool* fp32_tried = nullptr,
-                                                    const IV* a1_row = nullptr) {
+                                                    const IV* a1_row = nullptr, bool second_first = false) {
   IV r2 = r2_;
   if constexpr (is_3d(F)) r2.tighten_lb(r3.lb);
   IV r1 = r1_;
@@ -745,7 +775,7 @@ __device__ __forceinline__ uint8_t eval_kernel_body(IV la, IV r1_, IV r2_, IV r3
     }
   }
   const IV* a1 = (a1_row && dbits(r1.lb) == dbits(r1_.lb) && dbits(r1.ub) == dbits(r1_.ub)) ? a1_row : nullptr;
-  int c = criteria<F, double>(la, r1, r2, r3, a1);
+  int c = criteria<F, double>(la, r1, r2, r3, a1, second_first);
   if (c >= 16) return vbyte(V_TRAP, nofit, c - 16);
   if (c) return vbyte(V_CRITICAL, nofit, c);
   return vbyte(V_PROVEN, nofit, 0);
@@ -831,10 +861,11 @@ __device__ __forceinline__ IV r3_from_position(IV range, IV p) {
 }
 
 // Synthetic cell (la, r1, r2, p3) of family F; lbrsp = lower_bound_radius_single_placement(la.lb).
+// second_first: the learned check order (two-check families, wave-uniform).
 template <int F, bool PREFILTER = false>
 __device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp,
                                                   bool pre = true, bool* fp32_hit = nullptr,
-                                                  bool* fp32_tried = nullptr) {
+                                                  bool* fp32_tried = nullptr, bool second_first = false) {
   IV r1c = r1, r2c = r2;
   if (clip) {
     r1c = r1.intersect(compute_r1_range(la, lbrsp));
@@ -848,7 +879,7 @@ __device__ __forceinline__ uint8_t eval_synthetic(IV la, IV r1, IV r2, IV p3, bo
     if (range.empty()) return vbyte(V_OOD, 0, 0);
     r3 = r3_from_position(range, p3);
   }
-  return eval_kernel_body<F, PREFILTER>(la, r1c, r2c, r3, pre, fp32_hit, fp32_tried);
+  return eval_kernel_body<F, PREFILTER>(la, r1c, r2c, r3, pre, fp32_hit, fp32_tried, nullptr, second_first);
 }
 
 }  // namespace cpl

@@ -4,6 +4,9 @@
 // Loss per record (features x, targets t):
 //   BCE(y0, t0)                                  gate: "the FP32 prefilter proves the cell"
 //   + t5 * sum_{a=1..4} (y_a - t_a)^2 / 4        split-axis scores, CRITICAL cells only
+//   + BCE(y5, s)                                 check order: s = 1 when the record's FP64
+//                                                verdict byte t6 is CRITICAL with failing
+//                                                criterion 2 (its first check holds)
 // averaged over the batch.  Kernels:
 //   k_mlp_fwd_bwd   one thread per record: forward, loss, output and hidden
 //                   gradients (weights staged in LDS)
@@ -29,6 +32,7 @@ using cpl_host::fail;
 namespace {
 
 constexpr int REC_CHUNK = 512;  // records per weight-gradient partial
+constexpr int INFER_OUT = 6;    // outputs per record of k_mlp_infer (0..5)
 
 __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ X, const float* __restrict__ T, int n,
                                                      const float* __restrict__ W, float inv_n, float* __restrict__ A1,
@@ -80,8 +84,15 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ X
       loss += t[5] * d * d * 0.25f;
       dy[a] = t[5] * d * 0.5f * inv_n;
     }
+    {
+      const int vb = (int)t[6];
+      const float s5 = ((vb & 7) == 1 && ((vb >> 4) & 15) == 2) ? 1.0f : 0.0f;
+      const float y5 = y[5];
+      loss += fmaxf(y5, 0.0f) - y5 * s5 + log1pf(expf(-fabsf(y5)));
+      dy[5] = (1.0f / (1.0f + expf(-y5)) - s5) * inv_n;
+    }
 #pragma unroll
-    for (int k = 5; k < MLP_OUT; ++k) dy[k] = 0.0f;
+    for (int k = 6; k < MLP_OUT; ++k) dy[k] = 0.0f;
     loss *= inv_n;
     float dh2[MLP_H];
 #pragma unroll
@@ -180,11 +191,11 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const MlpPacked* __restrict__ 
   float f[MLP_IN];
 #pragma unroll
   for (int i = 0; i < MLP_IN; ++i) f[i] = b < n ? X[b * MLP_IN + i] : 0.0f;
-  float y[5];
+  float y[INFER_OUT];
   mlp_infer_wave(M, f, y);
   if (b < n)
 #pragma unroll
-    for (int k = 0; k < 5; ++k) out[b * 5 + k] = y[k];
+    for (int k = 0; k < INFER_OUT; ++k) out[b * INFER_OUT + k] = y[k];
 }
 
 struct MlpBufs {
@@ -360,7 +371,7 @@ int cpl_mlp_infer(const float* features, size_t n, const float* params, float* o
   int rc = cpl_host::ensure_init();
   if (rc) return rc;
   if (!features || !params || !out || n == 0) return fail(CPL_ERR_ARG, "cpl_mlp_infer: bad argument");
-  if ((rc = upload_records(features, nullptr, n)) || (rc = mb.Y.ensure(n * 5 * sizeof(float))) ||
+  if ((rc = upload_records(features, nullptr, n)) || (rc = mb.Y.ensure(n * INFER_OUT * sizeof(float))) ||
       (rc = mb.packed.ensure(sizeof(MlpPacked))))
     return rc;
   std::vector<char> host(sizeof(MlpPacked));
@@ -370,7 +381,7 @@ int cpl_mlp_infer(const float* features, size_t n, const float* params, float* o
   hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, mb.packed.as<MlpPacked>(),
                      mb.X.as<float>(), n, mb.Y.as<float>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(out, mb.Y.p, n * 5 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(out, mb.Y.p, n * INFER_OUT * sizeof(float), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return CPL_OK;
 }

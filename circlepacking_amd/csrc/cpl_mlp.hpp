@@ -3,8 +3,10 @@
 // No counterpart in the reference (whose kernels evaluate every cell in a
 // fixed order).  The network only proposes: output 0 is the logit of "the FP32
 // prefilter proves this cell", used to decide whether the FP32 screen runs
-// before the FP64 criteria; outputs 1..4 score the four split axes.  Every
-// verdict is still decided by the interval evaluation.
+// before the FP64 criteria; outputs 1..4 score the four split axes; output 5 is
+// the logit of "the second check of a two-check family fails first" (its
+// first check holds), used to order the checks.  Every verdict is still
+// decided by the interval evaluation.
 //
 // Parameters (FP32, this order): W1[8][64], b1[64], W2[64][64], b2[64],
 // W3[64][8], b3[8]; W[i][j] connects input i to unit j.
@@ -120,15 +122,15 @@ __device__ __forceinline__ float mlp_relu(float v) {
 }
 
 // One wave: every lane passes its cell's features (zeros for idle lanes) and
-// receives outputs 0..4 of its own cell.  Must be called by all 64 lanes.
-__device__ __forceinline__ void mlp_infer_wave(const MlpRegs& W, const float* f, float* out5) {
+// receives outputs 0..5 of its own cell.  Must be called by all 64 lanes.
+__device__ __forceinline__ void mlp_infer_wave(const MlpRegs& W, const float* f, float* out6) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   bf16x8 own;
 #pragma unroll
   for (int i = 0; i < 8; ++i) own[i] = (__bf16)f[i];
   const bf16x8 none = {};
-  float o[4] = {0.0f, 0.0f, 0.0f, 0.0f}, o4 = 0.0f;
+  float o[4] = {0.0f, 0.0f, 0.0f, 0.0f}, o4 = 0.0f, o5 = 0.0f;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     // layer 1 B operand: K slice j holds the features of lane group j's cells
@@ -161,19 +163,21 @@ __device__ __forceinline__ void mlp_infer_wave(const MlpRegs& W, const float* f,
       for (int i = 0; i < 8; ++i) b[s][i] = (__bf16)h2[2 * s + i / 4][i % 4];
     f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[j][0], b[0], W.b3[j], 0, 0, 0);
     y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[j][1], b[1], y, 0, 0, 0);
-    // outputs 0..3 of lane group j's cells sit in group j itself, output 4 in group (j + 1) mod 4
+    // outputs 0..3 of lane group j's cells sit in group j itself, outputs 4, 5 in group (j + 1) mod 4
 #pragma unroll
     for (int r = 0; r < 4; ++r) o[r] = g == j ? y[r] : o[r];
     o4 = g == ((j + 1) & 3) ? y[0] : o4;
+    o5 = g == ((j + 1) & 3) ? y[1] : o5;
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) out5[r] = o[r];
-  out5[4] = __shfl(o4, (lane + 16) & 63, 64);
+  for (int r = 0; r < 4; ++r) out6[r] = o[r];
+  out6[4] = __shfl(o4, (lane + 16) & 63, 64);
+  out6[5] = __shfl(o5, (lane + 16) & 63, 64);
 }
-__device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, const float* f, float* out5) {
+__device__ __forceinline__ void mlp_infer_wave(const MlpPacked* __restrict__ M, const float* f, float* out6) {
   MlpRegs W;
   mlp_load(M, W);
-  mlp_infer_wave(W, f, out5);
+  mlp_infer_wave(W, f, out6);
 }
 
 }  // namespace cpl

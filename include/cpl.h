@@ -102,10 +102,15 @@ typedef struct {
 } cpl_refine_cfg;
 
 /* Execution options of cpl_refine_run.  Every option except
- * CPL_OPT_LEARNED_SPLIT changes scheduling or the precision of intermediate
- * filtering only: verdicts, counts and root flags are identical to the plain
- * path.  CPL_OPT_LEARNED_SPLIT changes which cells are evaluated (another
- * bisection order); each verdict is still the interval evaluation's. */
+ * CPL_OPT_LEARNED_SPLIT and CPL_OPT_LEARNED_ORDER changes scheduling or the
+ * precision of intermediate filtering only: verdicts, counts and root flags
+ * are identical to the plain path.  CPL_OPT_LEARNED_SPLIT changes which cells
+ * are evaluated (another bisection order); each verdict is still the interval
+ * evaluation's.  CPL_OPT_LEARNED_ORDER changes the order of the two checks of
+ * the middle, end 2-disk and end 3-disk families: verdict kinds, counts, item
+ * digests and root flags stay the plain path's, and only the failing-criterion
+ * detail of a cell on which both checks fail (and a trap that the other order
+ * would not reach) can differ.  Neither has a counterpart in the reference. */
 enum {
   CPL_OPT_GRAPH = 1,       /* all rounds enqueued with device-side sizes, captured once as a hipGraph, replayed */
   CPL_OPT_PERSISTENT = 2,  /* one persistent kernel drains a mixed-depth queue (LDS-staged batches) */
@@ -113,6 +118,7 @@ enum {
   CPL_OPT_FP32 = 8,        /* FP32 outward-rounded prefilter; FP64 certifies what it leaves undecided */
   CPL_OPT_LEARNED = 16,    /* MLP proposal decides per cell whether the FP32 screen runs before the FP64 criteria */
   CPL_OPT_LEARNED_SPLIT = 32, /* mode-0 splits take the MLP's best-scored axis; changes the leaf set, not soundness */
+  CPL_OPT_LEARNED_ORDER = 64, /* a wave evaluates first the check its cells' proposals predict to fail first */
   /* The learned options apply from depth 1 on (the root round runs without a
    * proposal) and run with the plain and CPL_OPT_GRAPH forms. */
   CPL_OPT_NO_DIGEST = 256  /* skip the per-item digests (timing runs) */
@@ -355,10 +361,10 @@ int cpl_mlp_train(const float* features, const float* targets, size_t n, float* 
  * outputs (n x 8) of the training kernels over n records (gradient checks). */
 int cpl_mlp_grad(const float* features, const float* targets, size_t n, const float* params, float* grad_out,
                  float* loss_out, float* outputs);
-/* Outputs 0..4 (n x 5) of the in-kernel BF16 MFMA inference path. */
+/* Outputs 0..5 (n x 6) of the in-kernel BF16 MFMA inference path. */
 int cpl_mlp_infer(const float* features, size_t n, const float* params, float* out);
 /* Installs (params != NULL) or removes the proposal of a family for the
- * CPL_OPT_LEARNED / CPL_OPT_LEARNED_SPLIT runs. */
+ * CPL_OPT_LEARNED / CPL_OPT_LEARNED_SPLIT / CPL_OPT_LEARNED_ORDER runs. */
 int cpl_mlp_install(int family, const float* params);
 
 #ifdef __cplusplus

@@ -7,7 +7,9 @@ pinned by central finite differences in tests/test_oracle_mlp.py.
 
 Parameter vector (5256 floats): W1[8][64], b1[64], W2[64][64], b2[64],
 W3[64][8], b3[8]; W[i][j] connects input i to unit j.  Loss per record:
-BCE(y0, t0) + t5 * sum_{a=1..4} (y_a - t_a)^2 / 4, averaged over records.
+BCE(y0, t0) + t5 * sum_{a=1..4} (y_a - t_a)^2 / 4 + BCE(y5, s), averaged over
+records, where s = 1 when the FP64 verdict byte t6 is CRITICAL (kind 1) with
+failing criterion 2 (detail nibble 2), else 0.
 """
 import numpy as np
 
@@ -49,10 +51,15 @@ def loss_and_grad(x, t, w):
     y0, t0, m = y[:, 0], t[:, 0], t[:, 5]
     bce = np.maximum(y0, 0) - y0 * t0 + np.log1p(np.exp(-np.abs(y0)))
     d = y[:, 1:5] - t[:, 1:5]
-    loss = float(np.mean(bce + m * np.sum(d * d, axis=1) / 4.0))
+    vb = t[:, 6].astype(np.int64)
+    s5 = (((vb & 7) == 1) & (((vb >> 4) & 15) == 2)).astype(np.float64)
+    y5 = y[:, 5]
+    bce5 = np.maximum(y5, 0) - y5 * s5 + np.log1p(np.exp(-np.abs(y5)))
+    loss = float(np.mean(bce + m * np.sum(d * d, axis=1) / 4.0 + bce5))
     dy = np.zeros_like(y)
     dy[:, 0] = (1.0 / (1.0 + np.exp(-y0)) - t0) / n
     dy[:, 1:5] = m[:, None] * d / 2.0 / n
+    dy[:, 5] = (1.0 / (1.0 + np.exp(-y5)) - s5) / n
     g = {"W3": h2.T @ dy, "b3": dy.sum(0)}
     dh2 = (dy @ p["W3"].T) * (h2 > 0)
     g["W2"] = h1.T @ dh2
@@ -72,10 +79,10 @@ def bf16(a):
 
 
 def forward_bf16(x, w):
-    """Outputs 0..4 as the MFMA inference computes them: BF16 weights and
+    """Outputs 0..5 as the MFMA inference computes them: BF16 weights and
     activations, wide accumulation, FP32 biases."""
     p = unpack(np.asarray(w, dtype=np.float32))
     h1 = np.maximum(bf16(x) @ bf16(p["W1"]) + p["b1"], 0.0)
     h2 = np.maximum(bf16(h1) @ bf16(p["W2"]) + p["b2"], 0.0)
     y = bf16(h2) @ bf16(p["W3"]) + p["b3"]
-    return y[:, :5]
+    return y[:, :6]

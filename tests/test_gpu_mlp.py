@@ -18,6 +18,8 @@ def _records(seed, n):
     t[:, 0] = rng.integers(0, 2, n)
     t[:, 1:5] = rng.choice([0.0, 0.5, 1.0], (n, 4))
     t[:, 5] = rng.integers(0, 2, n)
+    # verdict bytes: PROVEN, CRITICAL at criterion 1 or 2, out of domain
+    t[:, 6] = rng.choice([0, 1 | 16, 1 | 32, 4], n)
     return x, t
 
 
@@ -47,7 +49,7 @@ def test_mfma_inference_matches_bf16_emulation(cpl):
     assert np.max(np.abs(got - ref)) <= 2e-3 * scale, np.max(np.abs(got - ref))
     # and close to the FP32 network (BF16 rounding only)
     _, _, y = mlp_ref.forward(x, w.astype(np.float64))
-    assert np.max(np.abs(got - y[:, :5])) <= 5e-2 * scale
+    assert np.max(np.abs(got - y[:, :6])) <= 5e-2 * scale
 
 
 def test_training_converges(cpl):
@@ -145,6 +147,54 @@ def test_learned_split_leaves_less_unresolved_volume(cpl, trained):
     print(f"unresolved volume: widest-axis {v_plain:.6g} (depth {base.max_depth}), {v_deep:.6g} (one deeper); "
           f"learned split {v_learned:.6g}")
     assert v_learned < 0.9 * v_plain, (v_learned, v_plain)
+
+
+_KIND_FIELDS = ("rounds", "evaluations", "verdicts", "digest", "family_verdicts", "total_evaluations", "certified",
+                "unresolved")
+
+
+def _same_kinds(a, b):
+    """Verdict kinds, counts, item digests equal (the failing-criterion detail,
+    in the verdict digest, may differ under a learned check order)."""
+    da, db = a.as_dict(), b.as_dict()
+    for k in _KIND_FIELDS:
+        assert da[k] == db[k], k
+
+
+@pytest.mark.parametrize("opts", [0, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS])
+def test_learned_order_keeps_verdict_kinds(cpl, trained, opts):
+    base = _small()
+    plain_s, plain_f = cpl.refine_run(base)
+    s, f = cpl.refine_run(abi.with_options(base, opts | abi.OPT_LEARNED_ORDER))
+    _same_kinds(s, plain_s)
+    assert np.array_equal(f, plain_f)
+
+
+def test_second_check_first_everywhere_keeps_verdict_kinds(cpl, trained):
+    # a proposal whose check-order logit is +20 on every cell: every wave of the
+    # two-check families runs its second check first; kinds, counts, item
+    # digests and root flags stay the plain path's, and the failing-criterion
+    # details change where both checks fail
+    w = learned.init_params(5)
+    w[abi.MLP_W3:abi.MLP_B3].reshape(abi.MLP_H, 8)[:, 5] = 0.0
+    w[abi.MLP_B3 + 5] = 20.0
+    base = abi.cfg_4param(grid=(16, 16, 16, 16), families=(abi.MIDDLE, abi.END_3D))
+    for fam in range(6):
+        cpl.mlp_install(fam, w)
+    try:
+        plain_s, plain_f = cpl.refine_run(base)
+        for opts in (0, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS):
+            s, f = cpl.refine_run(abi.with_options(base, opts | abi.OPT_LEARNED_ORDER))
+            _same_kinds(s, plain_s)
+            assert np.array_equal(f, plain_f)
+            assert s.as_dict()["verdict_digest"] != plain_s.as_dict()["verdict_digest"]
+    finally:  # the module's trained proposals again, as learned.train installed them
+        for fam in range(6):
+            if fam not in trained:
+                cpl.mlp_install(fam, None)
+            else:
+                params = trained[fam][0]
+                cpl.mlp_install(fam, params if params is not None else learned.init_params())
 
 
 def test_learned_rejected_with_persistent_form(cpl, trained):

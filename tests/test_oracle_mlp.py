@@ -12,6 +12,8 @@ def _data(seed, n):
     t[:, 0] = rng.integers(0, 2, n)
     t[:, 1:5] = rng.choice([0.0, 0.5, 1.0], (n, 4))
     t[:, 5] = rng.integers(0, 2, n)
+    # verdict bytes: PROVEN, CRITICAL at criterion 1 or 2, out of domain
+    t[:, 6] = rng.choice([0, 1 | 16, 1 | 32, 4], n)
     w = rng.normal(0, 0.4, mlp_ref.NPARAM)
     return x, t, w
 
@@ -35,11 +37,16 @@ def test_loss_closed_forms():
     x, t, _ = _data(3, 25)
     w = np.zeros(mlp_ref.NPARAM)
     loss, g = mlp_ref.loss_and_grad(x, t, w)
-    # zero network: y = 0, BCE = log 2, squared error = sum t_a^2 / 4
-    want = np.mean(np.log(2.0) + t[:, 5] * np.sum(t[:, 1:5] ** 2, axis=1) / 4)
+    # zero network: y = 0, both BCE terms = log 2, squared error = sum t_a^2 / 4
+    want = np.mean(2 * np.log(2.0) + t[:, 5] * np.sum(t[:, 1:5] ** 2, axis=1) / 4)
     assert abs(loss - want) < 1e-12
     # bias gradient of the gate output: mean(sigmoid(0) - t0)
     assert abs(g[5248] - np.mean(0.5 - t[:, 0])) < 1e-12
+    # bias gradient of the check-order output: mean(sigmoid(0) - s), s = 1 only
+    # for CRITICAL at criterion 2 (verdict byte 1 | 2 << 4)
+    s = (t[:, 6] == (1 | 32)).astype(np.float64)
+    assert abs(g[5248 + 5] - np.mean(0.5 - s)) < 1e-12
+    assert 0 < s.sum() < len(s)
 
 
 def test_pack_roundtrip_and_layout():
