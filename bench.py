@@ -288,7 +288,7 @@ def main():
         print(text)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: this prover runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the proof over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
 

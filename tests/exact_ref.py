@@ -659,7 +659,7 @@ class Reference:
         """Verdict byte of one reference-grid cell (la, r1_, r2_, r3 before
         tightening), following the kernel bodies (start :126-162, :199-211,
         :279-293; middle :166-201; end :107-143, :225-264).  EMPTY for every
-        family follows the verdict definition of this build (the reference checks
+        family follows the library's verdict definition (the reference checks
         emptiness in end 3d only, end :232-234; elsewhere a tightened interval of
         the reference grids cannot be empty)."""
         r2 = r2_.copy()
@@ -716,7 +716,7 @@ class Reference:
         finally:
             self.record = None
 
-    # ---- synthetic cells (the 4-parameter workload of this build) -------------------
+    # ---- synthetic cells (the 4-parameter refinement workload) -----------------------
     @staticmethod
     def r3_from_position(rng, p):
         """r3 = the position interval p in [0, 1] mapped onto the admissible r3
