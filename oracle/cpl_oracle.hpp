@@ -37,6 +37,8 @@
 //                (test_oracle_criteria_soundness.py); planted mistakes are
 //                caught (test_oracle_mutants.py)
 //   ordered map  brute force over bit patterns (test_ordered_transform.py)
+//   check order  the same checks in the other order (second_first):
+//                kinds and counts unchanged (test_oracle_check_order.py)
 //   driver       synthetic trace test of the lambda queue rule (test_driver.py)
 #pragma once
 
@@ -972,7 +974,10 @@ static inline bool possibly_no_fit(IV la, IV r1) { return mul_ru(2.0, r1.ub) >= 
 // One cell of family f, from the kernel bodies: r1_ and r2_ are the grid
 // subintervals before tightening, r3 the innermost subinterval (unused for 2d
 // families).  Returns the verdict byte.
-static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3) {
+// second_first: the two-check families (middle, end 2-disk, end 3-disk) run
+// their second check first and report the first failing check in that order
+// (the library's learned check order; no counterpart in the reference).
+static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3, bool second_first = false) {
   IV r2 = r2_;
   if (family_is_3d(f)) r2.tighten_lb(r3.lb);
   IV r1 = r1_;
@@ -1001,16 +1006,19 @@ static inline uint8_t eval_kernel_body(int f, IV la, IV r1_, IV r2_, IV r3) {
       case START_END_3D:
         return handle_3d_case(la, r1, r2, r3) ? vbyte(PROVEN, 0, 0) : vbyte(CRITICAL, 0, 1);
       case MIDDLE:
+        if (second_first && !handle_case_2_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
         if (!handle_case_1_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 1);
-        if (!handle_case_2_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
+        if (!second_first && !handle_case_2_1(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
         return vbyte(PROVEN, 0, 0);
       case END_2D:
+        if (second_first && !handle_outer_inner_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 2);
         if (!handle_inner_outer_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 1);
-        if (!handle_outer_inner_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 2);
+        if (!second_first && !handle_outer_inner_2d(la, r1, r2)) return vbyte(CRITICAL, 0, 2);
         return vbyte(PROVEN, 0, 0);
       case END_3D:
+        if (second_first && !handle_r1_outer_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
         if (!handle_r1_inner_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 1);
-        if (!handle_r1_outer_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
+        if (!second_first && !handle_r1_outer_3d(la, r1, r2, r3)) return vbyte(CRITICAL, 0, 2);
         return vbyte(PROVEN, 0, 0);
     }
   } catch (const Trap& t) {
@@ -1084,7 +1092,8 @@ static inline IV r3_from_position(IV range, IV p) {
 // first intersected with f's admissible range (OOD if that is empty);
 // without clip the box is evaluated as given.  lbrsp is
 // lower_bound_radius_single_placement(la.lb).
-static inline uint8_t eval_synthetic(int f, IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp) {
+static inline uint8_t eval_synthetic(int f, IV la, IV r1, IV r2, IV p3, bool clip, double lbrsp,
+                                     bool second_first = false) {
   IV r1c = r1, r2c = r2;
   if (clip) {
     r1c = r1.intersect(compute_r1_range(la, lbrsp));
@@ -1098,7 +1107,7 @@ static inline uint8_t eval_synthetic(int f, IV la, IV r1, IV r2, IV p3, bool cli
     if (range.empty()) return vbyte(OOD, 0, 0);
     r3 = r3_from_position(range, p3);
   }
-  return eval_kernel_body(f, la, r1c, r2c, r3);
+  return eval_kernel_body(f, la, r1c, r2c, r3, second_first);
 }
 
 // Atomic_ordered_double's order-preserving map (atomic_float.hpp:56-66).

@@ -117,7 +117,11 @@ void process(RunState& st, LocalStats& ls, const Item& it, uint8_t* flags_out) {
     flop_count() = saved;
   }
   uint64_t f0 = flop_count();
-  uint8_t vb = eval_synthetic(it.family, la, r1, r2, p3, cfg->clip != 0, lbrsp);
+  // CPL_OPT_LEARNED_ORDER: the oracle's stand-in for a proposal that always
+  // votes for the second check (refinement rounds only; the root round has no
+  // proposal)
+  const bool second_first = (cfg->options & CPL_OPT_LEARNED_ORDER) && it.depth >= 1;
+  uint8_t vb = eval_synthetic(it.family, la, r1, r2, p3, cfg->clip != 0, lbrsp, second_first);
   int v = vb & 7;
   int d = it.depth;
   ls.flops[it.family][v] += flop_count() - f0;

@@ -170,11 +170,12 @@ def test_learned_order_keeps_verdict_kinds(cpl, trained, opts):
     assert np.array_equal(f, plain_f)
 
 
-def test_second_check_first_everywhere_keeps_verdict_kinds(cpl, trained):
+def test_second_check_first_everywhere_keeps_verdict_kinds(cpl, cplo, trained):
     # a proposal whose check-order logit is +20 on every cell: every wave of the
-    # two-check families runs its second check first; kinds, counts, item
-    # digests and root flags stay the plain path's, and the failing-criterion
-    # details change where both checks fail
+    # two-check families runs its second check first in the refinement rounds;
+    # kinds, counts, item digests and root flags stay the plain path's, the
+    # failing-criterion details change where both checks fail, and everything,
+    # details included, equals the oracle evaluating the second check first
     w = learned.init_params(5)
     w[abi.MLP_W3:abi.MLP_B3].reshape(abi.MLP_H, 8)[:, 5] = 0.0
     w[abi.MLP_B3 + 5] = 20.0
@@ -183,11 +184,14 @@ def test_second_check_first_everywhere_keeps_verdict_kinds(cpl, trained):
         cpl.mlp_install(fam, w)
     try:
         plain_s, plain_f = cpl.refine_run(base)
+        ref_s, ref_f = cplo.refine_run(abi.with_options(base, abi.OPT_LEARNED_ORDER))
         for opts in (0, abi.OPT_GRAPH | abi.OPT_HIER_ROOTS):
             s, f = cpl.refine_run(abi.with_options(base, opts | abi.OPT_LEARNED_ORDER))
             _same_kinds(s, plain_s)
             assert np.array_equal(f, plain_f)
             assert s.as_dict()["verdict_digest"] != plain_s.as_dict()["verdict_digest"]
+            assert s.as_dict() == ref_s.as_dict()
+            assert np.array_equal(f, ref_f)
     finally:  # the module's trained proposals again, as learned.train installed them
         for fam in range(6):
             if fam not in trained:
