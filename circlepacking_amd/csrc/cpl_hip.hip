@@ -1546,6 +1546,7 @@ struct Plan {
   bool flags = false;
   bool ext = false;       // extended kernels (FP32 prefilter)
   bool learned = false;
+  bool last_round_proposals = false;  // the last round reads proposals (gate or check order; not the split axis)
   uint32_t prefilter_depths = 0;
   int block = 256;
   int cpt = 1;
@@ -1578,6 +1579,7 @@ int make_plan(const cpl_refine_cfg* cfg, bool want_flags, Plan& pl) {
   pl.digest = !(cfg->options & CPL_OPT_NO_DIGEST);
   pl.flags = want_flags;
   pl.learned = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_SPLIT | CPL_OPT_LEARNED_ORDER)) != 0;
+  pl.last_round_proposals = (cfg->options & (CPL_OPT_LEARNED | CPL_OPT_LEARNED_ORDER)) != 0;
   pl.ext = (cfg->options & CPL_OPT_FP32) != 0;
   pl.prefilter_depths = !(cfg->options & CPL_OPT_FP32) ? 0u : cfg->prefilter_depths ? cfg->prefilter_depths : 0xFFFFFFFFu;
   if (pl.learned)
@@ -1674,6 +1676,9 @@ constexpr uint64_t kProposeGroups = 8;
 void launch_propose(const Plan& pl, int s, RefineParams& P, const Worklist& in, uint64_t n, uint64_t n_expect) {
   P.prop = nullptr;
   if (!pl.learned) return;
+  // the last round splits nothing: with the split axis the only use of the
+  // proposal, its round needs none
+  if (P.depth >= P.max_depth && !pl.last_round_proposals) return;
   uint8_t* out = g.prop.as<uint8_t>() + (size_t)s * pl.cap;
   const uint64_t per_block = 256 * kProposeGroups;
   const uint64_t cap = (uint64_t)std::max(g.cu_count, 1) * 8;
