@@ -287,7 +287,7 @@ __device__ __forceinline__ int split_axis(const RefineParams& P, uint32_t meta, 
 
 // Proposal bytes of a round's input items (before the round runs on them).
 // Each wave loads the network's operands into registers once and then walks
-// 64-cell groups with a wave-level grid stride (the operands are ~23 KB per
+// 64-cell groups with a wave-level grid stride (the operands are 32 KB per
 // wave; reloading them per group dominated the kernel's time).
 // Two waves per SIMD (at most 256 registers): the MFMA results then go to
 // vector registers directly instead of accumulation registers read back one

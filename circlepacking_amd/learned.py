@@ -2,9 +2,10 @@
 per-family MLP on the GPU, and install it for CPL_OPT_LEARNED runs.
 
 No counterpart in the reference.  The proposal only decides whether the FP32
-prefilter runs before the FP64 criteria on a cell (output 0) and scores split
-axes (outputs 1..4, used by CPL_OPT_LEARNED_SPLIT); every verdict comes from
-the interval evaluation.
+prefilter runs before the FP64 criteria on a cell (output 0), scores split
+axes (outputs 1..4, used by CPL_OPT_LEARNED_SPLIT) and predicts that the second
+check of a two-check family fails first (output 5, used by
+CPL_OPT_LEARNED_ORDER); every verdict comes from the interval evaluation.
 """
 import numpy as np
 
