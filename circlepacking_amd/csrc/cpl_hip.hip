@@ -1346,9 +1346,15 @@ const char* cpl_last_error(void) { return g_err.c_str(); }
 
 int cpl_init(void) { return ensure_init(); }
 
+namespace {
+void drop_graphs();
+}
+
 int cpl_shutdown(void) {
   if (!g.ready) return CPL_OK;
   (void)hipStreamSynchronize(g.stream);
+  drop_graphs();  // their kernels address the buffers released below
+  for (bool& r : g.mlp_ready) r = false;  // the proposals live in g.mlp
   for (DevBuf* b : {&g.scratch_in, &g.scratch_in2, &g.scratch_out, &g.cells, &g.verdicts, &g.wl[0], &g.wl[1],
                     &g.la_table, &g.root_flags, &g.stats, &g.stats_sum, &g.counters, &g.mr_ranges, &g.mr_misc, &g.mr_crit,
                     &g.persist_q, &g.mlp, &g.records, &g.prop})
@@ -1895,6 +1901,10 @@ struct GraphCache {
   std::vector<uint32_t> expect;
   // end of each family chain in the previous replay (ms after its start; 0: not measured)
   float chain_ms[CPL_NUM_FAMILIES] = {};
+  // device buffers the captured kernels address: a buffer that another run
+  // reallocated (or cpl_shutdown released) forces a new capture
+  std::vector<void*> bufs;
+  uint64_t cap = 0;  // worklist capacity per family region (CPL_WORKLIST_CAP) the grids and clamps were built for
   void reset() {
     for (int k = 0; k < 1 + CPL_NUM_FAMILIES; ++k) {
       if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
@@ -1907,6 +1917,15 @@ struct GraphCache {
   }
 };
 GraphCache g_graph;
+
+std::vector<void*> graph_buffers() {
+  return {g.wl[0].p, g.wl[1].p, g.la_table.p, g.root_flags.p, g.stats.p, g.stats_sum.p, g.counters.p, g.prop.p, g.mlp.p};
+}
+
+void drop_graphs() {
+  g_graph.reset();
+  g_graph.bufs.clear();
+}
 
 
 unsigned grid_tuned(uint64_t expect, const Plan& pl) {
@@ -1991,7 +2010,8 @@ void enqueue_family(const cpl_refine_cfg* cfg, const Plan& pl, int s, const std:
 }
 
 int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* stats, uint8_t* root_flags) {
-  bool same = g_graph.valid && g_graph.flags == pl.flags && std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) == 0;
+  bool same = g_graph.valid && g_graph.flags == pl.flags && std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) == 0 &&
+              g_graph.cap == pl.cap && g_graph.bufs == graph_buffers();
   if (!same) {
     // chain times of another configuration do not order this one's chains
     if (std::memcmp(&g_graph.cfg, cfg, sizeof(*cfg)) != 0)
@@ -2010,6 +2030,8 @@ int refine_graph(const cpl_refine_cfg* cfg, const Plan& pl, cpl_refine_stats* st
     g_graph.n = pl.nf;
     g_graph.cfg = *cfg;
     g_graph.flags = pl.flags;
+    g_graph.bufs = graph_buffers();
+    g_graph.cap = pl.cap;
     g_graph.valid = true;
   }
   int rc = ensure_stage();

@@ -105,3 +105,21 @@ def test_graph_runs_report_only_their_families(cpl, cplo):
             assert s.as_dict() == plain[0].as_dict()
             if flags:
                 assert np.array_equal(f, plain[1])
+
+
+def test_graph_recaptures_after_buffers_move(cpl):
+    """A captured graph addresses the library's device buffers: a plain run that
+    grows one of them (a deeper lambda table), and a shutdown that releases
+    them all, must make the next graph run capture again (its results equal the
+    first run's) instead of replaying kernels on freed memory."""
+    cfg = abi.with_options(abi.cfg_4param(grid=(16, 16, 16, 16), max_depth=4), G | H)
+    first = cpl.refine_run(cfg)
+    cpl.refine_run(cfg)  # a replay
+    # plain form with a lambda table larger than any earlier run's (64 cells x 2^11 levels):
+    # the table's buffer is reallocated
+    cpl.refine_run(abi.cfg_4param(grid=(64, 2, 2, 2), max_depth=10))
+    same(cpl.refine_run(cfg), first)
+    assert cpl.lib().cpl_shutdown() == 0
+    assert cpl.lib().cpl_init() == 0
+    same(cpl.refine_run(cfg), first)
+    same(cpl.refine_run(cfg), first)
