@@ -207,6 +207,8 @@ def summarize_graph_pmc_by_kernel(dirs):
             with open(path) as fh:
                 for row in csv.DictReader(fh):
                     name = row["Kernel_Name"].split("(")[0].split("<")[0].strip()
+                    if name.startswith("void "):
+                        name = name[len("void "):]
                     if name.startswith("__amd_rocclr"):
                         continue
                     k = tot.setdefault(name, {})
