@@ -158,6 +158,10 @@ typedef struct {
 /* Create the library context on the current HIP device (no counterpart in the
  * reference, which allocates per driver call: run_kernel.cuh:64). */
 int cpl_init(void);
+/* Release the context's device buffers, streams and events, the captured
+ * graphs of the graph form and the installed proposals (install them again
+ * after the next cpl_init); the no-allocation scratch stays (it is a device
+ * array of the library). */
 int cpl_shutdown(void);
 /* Message of the last failing call (thread-local, never NULL). */
 const char* cpl_last_error(void);
