@@ -494,7 +494,7 @@ def main():
             "cores": threads,
             "host_cores": os.cpu_count(),
             "kind": "port",
-            "sample": f"every {args.cpu_stride}th root cell of the same {args.grid}^4 grid ({ost.total_evaluations} "
+            "sample": f"one root cell in {args.cpu_stride} of the same {args.grid}^4 grid ({ost.total_evaluations} "
                       f"evaluations, {ost.certified} certified leaves, {c1 - c0:.1f} s wall)",
         }
         # the same sample is a parity check of the GPU's per-(root, family) outcome flags
